@@ -1,0 +1,149 @@
+/*
+ * sparc_hip.h -- C ABI of the MI355X (gfx950) SPARC2 time-step library.
+ *
+ * The reference's hot path is a set of argument-less Fortran module
+ * procedures working on module globals:
+ *   STEP               step.f90:20-63          (called at driver.f90:157)
+ *   MP_QUIET_3D        RHS.f90:18-175          (called at step.f90:40)
+ *   MP_MHD_3D          RHS.f90:363-668         (called at step.f90:44)
+ *   ddx / ddy / ddz    derivatives.f90:10,226,451
+ *   filtx/filty/filtz  derivatives.f90:119,345,565
+ *   FILTER_VARS        derivatives.f90:659-703 (called at driver.f90:200)
+ *   norm + guards      all_modules.f90:684-706, driver.f90:161-183
+ * This header replaces them with a context object that owns the device
+ * copies of everything those routines read or write.  All host arrays are
+ * FP64 in the reference's Fortran order (x fastest) and are borrowed only
+ * for the duration of a call.  Every entry point returns SPARC_OK (0) or a
+ * negative error code; sparc_hip_strerror() names the code and
+ * sparc_hip_last_error() gives the message of the most recent failure.
+ * Calls on one context must be serialised by the caller.
+ */
+#ifndef SPARC_HIP_H
+#define SPARC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPARC_OK 0
+#define SPARC_E_INVALID (-1) /* bad argument or configuration */
+#define SPARC_E_HIP (-2)     /* HIP runtime error (no device, launch failure, ...) */
+#define SPARC_E_NOMEM (-3)   /* device allocation failed */
+#define SPARC_E_STATE (-4)   /* call out of order (e.g. step before background) */
+
+#define SPARC_PREC_FP64 0 /* plain FP64 path */
+#define SPARC_PREC_FP32 1 /* FP32 fields/stencils/CPML, compensated RK update */
+
+/* PML faces, the order of psi arguments below. */
+#define SPARC_XBOT 0
+#define SPARC_XTOP 1
+#define SPARC_YBOT 2
+#define SPARC_YTOP 3
+#define SPARC_ZBOT 4
+#define SPARC_ZTOP 5
+
+typedef struct sparc_ctx sparc_ctx;
+
+/* Run geometry and switches.  Values are the ones the reference derives in
+ * READ_SPARC_PARAMETERS / INITANDALLOCATE / INITIALIZE_STEP. */
+typedef struct sparc_cfg {
+  int nx, ny, nz;        /* global grid (nx, ny, nz >= 16) */
+  int option;            /* 1 = MP_QUIET_3D (quiet/thermal), 3 = MP_MHD_3D;
+                            initialize.f90:1343-1363 */
+  int precision;         /* SPARC_PREC_FP64 or SPARC_PREC_FP32 */
+  int periodic;          /* PERIODIC: x/y periodic */
+  int use_pml, use_hpml; /* USE_PML (z faces), USE_HPML (x/y faces) */
+  int npml[6];           /* nxpmlbot, nxpmltop, nypmlbot, nypmltop, nzpmlbot, nzpmltop */
+  int g_xy_filt, g_z_filt; /* filter half-widths, 5 or 3 */
+  double s_xy, s_z;      /* filter strengths (used when g = 3) */
+  int xyfreq, zfreq;     /* filter cadences in steps, initialize.f90:1911-1912 */
+  int pulse_dir;         /* 0 no source, 1 vx, 2 vy, 3 vz */
+  int bg3d;              /* 1: background arrays are nx*ny*nz; 0: nz-long z profiles
+                            (quiet backgrounds, all_modules.f90:353-384) */
+  double deltat;         /* nondimensional dt, initialize.f90:733 */
+  double betas[5];       /* RK stage weights x deltat, initialize.f90:1366-1381 */
+  double stretchx, stretchy; /* initialize.f90:503-517 */
+  const double* stretch; /* nz values, initialize.f90:728-730 */
+  int device;            /* HIP device ordinal */
+  int tile[3];           /* stage tile (x, y, z-chunk); 0 = library choice */
+} sparc_cfg;
+
+/* Backgrounds after INITIALISE_RHS (initialize.f90:883-1228), nondimensional.
+ * The first group is nz-long when cfg.bg3d == 0, else nx*ny*nz. */
+typedef struct sparc_bg {
+  const double *rho0, *rhoinv, *c2rho0, *gradrho0_z, *gradp0_z, *spongexyz, *p0;
+  const double* g; /* nz, PML-scaled gravity */
+  /* MHD only (nx*ny*nz): */
+  const double *box, *boy, *boz, *curlbox, *curlboy, *curlboz;
+  const double *gradrho0_x, *gradrho0_y, *gradp0_x, *gradp0_y, *reduction;
+  /* CPML coefficients az/bzpml/kappaz & co. per face (face-shaped, x fastest):
+   * x faces (npml, ny, nz), y faces (nx, npml, nz), z faces (nx, ny, npml).
+   * NULL for inactive faces. */
+  const double *pml_a[6], *pml_b[6], *pml_kappa[6];
+  /* Separable pulse profile: forcing_p(i,j,k) = amp(t) * src_x[i] * src_y[j] *
+   * src_z[k] (driver.f90:140-145); NULL when pulse_dir == 0. */
+  const double *src_x, *src_y, *src_z;
+  double dimc, dimrho; /* scales of the KE norm (driver.f90:161) */
+} sparc_bg;
+
+/* Context lifetime. */
+int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out);
+int sparc_hip_destroy(sparc_ctx* ctx);
+
+/* Upload the post-set-up backgrounds (replaces the module arrays STEP reads). */
+int sparc_hip_set_background(sparc_ctx* ctx, const sparc_bg* bg);
+
+/* State in / out: a is nv*nx*ny*nz (field-major, the reference's `a`,
+ * all_modules.f90:104); psi[f] is npsi*face_size for each active face (the
+ * PML bases zpmlvars/xpmlvars/ypmlvars), NULL for inactive faces or to leave
+ * them untouched (put) / skip them (get).  put_state clears the stage buffers. */
+int sparc_hip_put_state(sparc_ctx* ctx, const double* a, const double* const psi[6]);
+int sparc_hip_get_state(sparc_ctx* ctx, double* a, double* const psi[6]);
+
+/* STEP (step.f90:20-63) with the source amplitude of this step,
+ * amp = pulse_amp/0.56458272/dimc * sin(..) * exp(..) at t = time*timestep
+ * (driver.f90:140-142).  Asynchronous on the context's stream. */
+int sparc_hip_step(sparc_ctx* ctx, double src_amp);
+
+/* FILTER_VARS (derivatives.f90:659-703) for time index `time`. */
+int sparc_hip_filter(sparc_ctx* ctx, int time);
+
+/* The driver loop body for nn = n0..n1 (driver.f90:123-200): STEP with
+ * src_amp[nn-n0], then FILTER_VARS when nn != 0.  Asynchronous. */
+int sparc_hip_advance(sparc_ctx* ctx, int n0, int n1, const double* src_amp);
+
+/* Diagnostics of the most recent step: out[0] = norm_KE in cgs
+ * (driver.f90:161), out[1] = min(rho+rho0), out[2] = min(p+p0) over the
+ * stage-5 state (driver.f90:180).  Synchronises. */
+int sparc_hip_diag(sparc_ctx* ctx, double out[3]);
+
+/* Drop-in operators on one host FP64 array of nx*ny*nz (the context supplies
+ * the metric).  bc: 1 (one-sided edges) or 5 (periodic) for x/y; 1 for z. */
+int sparc_hip_ddx(sparc_ctx* ctx, const double* in, double* out, int bc);
+int sparc_hip_ddy(sparc_ctx* ctx, const double* in, double* out, int bc);
+int sparc_hip_ddz(sparc_ctx* ctx, const double* in, double* out, int bc);
+int sparc_hip_filtx(sparc_ctx* ctx, const double* in, double* out);
+int sparc_hip_filty(sparc_ctx* ctx, const double* in, double* out);
+int sparc_hip_filtz(sparc_ctx* ctx, const double* in, double* out);
+
+/* Wait for all work queued on the context. */
+int sparc_hip_sync(sparc_ctx* ctx);
+
+/* Kernel timing with HIP events on the context's stream: enable != 0 starts
+ * recording every stage launch; sparc_hip_kernel_time returns the number of
+ * timed launches and their total milliseconds for kind 0 = RK stages 1-4,
+ * 1 = RK stage 5, 2 = filter passes. */
+int sparc_hip_timing(sparc_ctx* ctx, int enable);
+int sparc_hip_kernel_time(sparc_ctx* ctx, int kind, long* count, double* total_ms);
+
+/* Tile (x, y, z-chunk) the stage kernel uses and its launch size. */
+int sparc_hip_tile(sparc_ctx* ctx, int tile[3], long* nblocks);
+
+const char* sparc_hip_strerror(int code);
+const char* sparc_hip_last_error(void);
+int sparc_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPARC_HIP_H */

@@ -1,0 +1,702 @@
+// api.hip -- host side of the C ABI declared in include/sparc_hip.h.
+//
+// A context owns every device buffer the reference's STEP / FILTER_VARS read
+// or write (module arrays of all_modules.f90:96-146).  State-sized buffers
+// form a pool of four (base state, its ping-pong partner, two stage RHS
+// buffers); their roles are swapped instead of copied.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sparc_hip.h"
+#include "sparc_dev.h"
+
+namespace sparc {
+template <typename T>
+hipError_t launch_stage(bool mhd, bool bg3d, int tx, int ty, const StageArgs<T>& A, hipStream_t s);
+bool stage_tile_supported(int tx, int ty);
+template <typename T>
+hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
+                         int nfield, int periodic, T s, int bare, hipStream_t st);
+hipError_t launch_deriv(int axis, const double* in, double* out, int nx, int ny, int nz, double sxy,
+                        const double* stretch, int bc, hipStream_t st);
+hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, hipStream_t st);
+hipError_t launch_zero(void* p, long n, bool f32, hipStream_t st);
+}  // namespace sparc
+
+using namespace sparc;
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(SPARC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+enum BgSlot {
+  B_RHO0, B_RHOINV, B_C2RHO0, B_GR0Z, B_GP0Z, B_SPONGE, B_P0,
+  B_BOX, B_BOY, B_BOZ, B_CBOX, B_CBOY, B_CBOZ, B_GR0X, B_GR0Y, B_GP0X, B_GP0Y, B_RED,
+  B_COUNT
+};
+
+struct FaceDev {
+  int active = 0, n = 0, off = 0;
+  long fsize = 0;
+  void *a = nullptr, *b = nullptr, *k = nullptr, *psi = nullptr, *kpsi = nullptr;
+};
+
+struct TimeRec {
+  int kind;
+  hipEvent_t e0, e1;
+};
+
+}  // namespace
+
+struct sparc_ctx {
+  sparc_cfg cfg{};
+  std::vector<double> stretch_h;
+  int dev = 0;
+  hipStream_t st = nullptr;
+  bool f32 = false, mhd = false, bg3d = false;
+  int nv = 5, npsi = 2;
+  int nx = 0, ny = 0, nz = 0;
+  long N = 0, fs = 0;
+  size_t esz = 8;
+  void* pool[4] = {nullptr, nullptr, nullptr, nullptr};
+  int ra = 0, ra2 = 1, rk0 = 2, rk1 = 3;
+  void* comp[2] = {nullptr, nullptr};
+  int rc = 0;
+  void* bg[B_COUNT] = {};
+  void *g = nullptr, *stretch = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
+  double* stretch_d = nullptr;  // FP64 copy for the standalone operators
+  FaceDev face[6];
+  double* partial = nullptr;
+  double* diag = nullptr;
+  double dimc = 1.0, dimrho = 1.0;
+  int tx = 64, ty = 8, kc = 8, ntx = 1, nty = 1, nchunk = 1;
+  bool bg_set = false, state_set = false;
+  long steps_done = 0;
+  double* op_buf = nullptr;  // 2*N doubles for the standalone operators
+  bool timing = false;
+  std::vector<TimeRec> trecs;
+  long tcount[3] = {0, 0, 0};
+  double tms[3] = {0, 0, 0};
+};
+
+static long round_up(long v, long m) { return (v + m - 1) / m * m; }
+
+static int dalloc(void** p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(SPARC_E_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " +
+                                   hipGetErrorString(e));
+  }
+  return SPARC_OK;
+}
+
+// Upload n doubles converted to the context's element type.
+static int upload(sparc_ctx* c, void* dst, const double* src, long n) {
+  if (n <= 0) return SPARC_OK;
+  if (!c->f32) {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  } else {
+    std::vector<float> tmp(n);
+    for (long q = 0; q < n; ++q) tmp[q] = (float)src[q];
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
+  return SPARC_OK;
+}
+
+static int download(sparc_ctx* c, double* dst, const void* src, long n) {
+  if (n <= 0) return SPARC_OK;
+  if (!c->f32) {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (long q = 0; q < n; ++q) dst[q] = (double)tmp[q];
+  }
+  return SPARC_OK;
+}
+
+static void free_all(sparc_ctx* c) {
+  auto F = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  for (auto& p : c->pool) F(p);
+  for (auto& p : c->comp) F(p);
+  for (auto& p : c->bg) F(p);
+  F(c->g);
+  F(c->stretch);
+  F(c->sx);
+  F(c->sy);
+  F(c->sz);
+  F(c->stretch_d);
+  F(c->partial);
+  F(c->diag);
+  F(c->op_buf);
+  for (auto& f : c->face) {
+    F(f.a);
+    F(f.b);
+    F(f.k);
+    F(f.psi);
+    F(f.kpsi);
+  }
+  for (auto& r : c->trecs) {
+    (void)hipEventDestroy(r.e0);
+    (void)hipEventDestroy(r.e1);
+  }
+  if (c->st) (void)hipStreamDestroy(c->st);
+}
+
+// Tile choice: the x extent covers a 64-wide wavefront row when the grid is
+// wide enough; the z chunk is sized so the launch has >= ~2048 blocks
+// (8 per CU on 256 CUs) without dropping below 4 planes per block.
+static void choose_tile(sparc_ctx* c) {
+  const sparc_cfg& q = c->cfg;
+  int tx = q.tile[0], ty = q.tile[1];
+  if (!(tx > 0 && ty > 0 && stage_tile_supported(tx, ty))) {
+    // MHD keeps 256-thread blocks: its register queue needs the 512-entry
+    // register file a single wave per SIMD gets.
+    if (c->nx >= 64 && c->nx % 64 == 0 && (q.periodic || !c->mhd)) {
+      tx = 64;
+      ty = c->mhd ? 4 : 8;
+    } else {
+      tx = 32;
+      ty = 8;
+    }
+  }
+  c->tx = tx;
+  c->ty = ty;
+  c->ntx = (c->nx + tx - 1) / tx;
+  c->nty = (c->ny + ty - 1) / ty;
+  long tiles = (long)c->ntx * c->nty;
+  int kc = q.tile[2];
+  if (kc <= 0) {
+    long want = 2048;
+    kc = (int)std::max<long>(4, (c->nz * tiles + want - 1) / want);
+    kc = std::min(kc, c->nz);
+  }
+  c->kc = kc;
+  c->nchunk = (c->nz + kc - 1) / kc;
+}
+
+static int check_tile(sparc_ctx* c) {
+  if (!stage_tile_supported(c->tx, c->ty))
+    return fail(SPARC_E_INVALID, "unsupported stage tile");
+  if (!c->cfg.periodic) {
+    // one-sided closures need rows 0..10 and n-11..n-1 inside one LDS tile
+    if (c->ty < 6) return fail(SPARC_E_INVALID, "non-periodic runs need tile_y >= 6");
+    int rx = c->nx % c->tx, ry = c->ny % c->ty;
+    if ((rx > 0 && rx < 6) || (ry > 0 && ry < 6))
+      return fail(SPARC_E_INVALID, "non-periodic runs need the last tile to hold >= 6 cells");
+  }
+  return SPARC_OK;
+}
+
+// The exported functions keep the C linkage of their declarations in
+// sparc_hip.h.
+
+int sparc_hip_version(void) { return 1; }
+
+const char* sparc_hip_last_error(void) { return g_last_error.c_str(); }
+
+const char* sparc_hip_strerror(int code) {
+  switch (code) {
+    case SPARC_OK: return "ok";
+    case SPARC_E_INVALID: return "invalid argument or configuration";
+    case SPARC_E_HIP: return "HIP runtime error";
+    case SPARC_E_NOMEM: return "device out of memory";
+    case SPARC_E_STATE: return "call out of order";
+    default: return "unknown error";
+  }
+}
+
+int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
+  if (!cfg || !out) return fail(SPARC_E_INVALID, "null argument");
+  *out = nullptr;
+  const sparc_cfg& q = *cfg;
+  if (q.nx < 16 || q.ny < 16 || q.nz < 16) return fail(SPARC_E_INVALID, "grid needs >= 16 cells per axis");
+  if (q.option != 1 && q.option != 3) return fail(SPARC_E_INVALID, "option must be 1 (quiet) or 3 (MHD)");
+  if (q.precision != SPARC_PREC_FP64 && q.precision != SPARC_PREC_FP32)
+    return fail(SPARC_E_INVALID, "precision must be FP64 or FP32");
+  if ((q.g_xy_filt != 5 && q.g_xy_filt != 3) || (q.g_z_filt != 5 && q.g_z_filt != 3))
+    return fail(SPARC_E_INVALID, "filter half-widths must be 5 or 3");
+  if (q.xyfreq < 1 || q.zfreq < 1) return fail(SPARC_E_INVALID, "filter cadences must be >= 1");
+  if (q.pulse_dir < 0 || q.pulse_dir > 3) return fail(SPARC_E_INVALID, "pulse_dir must be 0..3");
+  if (!q.stretch) return fail(SPARC_E_INVALID, "stretch (nz values) is required");
+  if (q.use_hpml && q.periodic) return fail(SPARC_E_INVALID, "USE_HPML requires PERIODIC off");
+  const int ext[3] = {q.nx, q.ny, q.nz};
+  for (int f = 0; f < 6; ++f) {
+    bool on = (f < 4 ? q.use_hpml : q.use_pml) && q.npml[f] > 0;
+    if (on && (q.npml[f] < 2 || 2 * q.npml[f] > ext[f / 2]))
+      return fail(SPARC_E_INVALID, "PML width must be >= 2 and fit the axis");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(SPARC_E_HIP, "no HIP device available");
+  if (q.device < 0 || q.device >= ndev) return fail(SPARC_E_INVALID, "bad device ordinal");
+
+  sparc_ctx* c = new sparc_ctx();
+  c->cfg = q;
+  c->stretch_h.assign(q.stretch, q.stretch + q.nz);
+  c->cfg.stretch = c->stretch_h.data();
+  c->dev = q.device;
+  c->f32 = q.precision == SPARC_PREC_FP32;
+  c->esz = c->f32 ? 4 : 8;
+  c->mhd = q.option == 3;
+  c->bg3d = c->mhd ? true : (q.bg3d != 0);
+  c->nv = c->mhd ? 8 : 5;
+  c->npsi = c->mhd ? 6 : 2;
+  c->nx = q.nx;
+  c->ny = q.ny;
+  c->nz = q.nz;
+  c->N = (long)q.nx * q.ny * q.nz;
+  c->fs = round_up(c->N, 256);
+  choose_tile(c);
+  int rc = check_tile(c);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  auto bail = [&](int code) {
+    free_all(c);
+    delete c;
+    return code;
+  };
+  if (hipSetDevice(c->dev) != hipSuccess) return bail(fail(SPARC_E_HIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(SPARC_E_HIP, "hipStreamCreate failed"));
+
+  const size_t sbytes = c->esz * (size_t)c->nv * (size_t)c->fs;
+  for (int p = 0; p < 4; ++p)
+    if ((rc = dalloc(&c->pool[p], sbytes))) return bail(rc);
+  if (c->f32)
+    for (int p = 0; p < 2; ++p)
+      if ((rc = dalloc(&c->comp[p], sbytes))) return bail(rc);
+  const long bgn = c->bg3d ? c->fs : q.nz;
+  const int nbg = c->mhd ? B_COUNT : B_P0 + 1;
+  for (int s = 0; s < nbg; ++s)
+    if ((rc = dalloc(&c->bg[s], c->esz * bgn))) return bail(rc);
+  if ((rc = dalloc(&c->g, c->esz * q.nz))) return bail(rc);
+  if ((rc = dalloc(&c->stretch, c->esz * q.nz))) return bail(rc);
+  if ((rc = dalloc(&c->sx, c->esz * q.nx))) return bail(rc);
+  if ((rc = dalloc(&c->sy, c->esz * q.ny))) return bail(rc);
+  if ((rc = dalloc(&c->sz, c->esz * q.nz))) return bail(rc);
+  if ((rc = dalloc((void**)&c->stretch_d, sizeof(double) * q.nz))) return bail(rc);
+  const long nblk = (long)c->ntx * c->nty * c->nchunk;
+  if ((rc = dalloc((void**)&c->partial, sizeof(double) * 3 * nblk))) return bail(rc);
+  if ((rc = dalloc((void**)&c->diag, sizeof(double) * 3))) return bail(rc);
+  for (int f = 0; f < 6; ++f) {
+    FaceDev& F = c->face[f];
+    F.active = (f < 4 ? q.use_hpml : q.use_pml) && q.npml[f] > 0;
+    if (!F.active) continue;
+    const int axis = f / 2;
+    F.n = q.npml[f];
+    F.off = (f % 2) ? ext[axis] - F.n : 0;
+    long d[3] = {q.nx, q.ny, q.nz};
+    d[axis] = F.n;
+    F.fsize = d[0] * d[1] * d[2];
+    if ((rc = dalloc(&F.a, c->esz * F.fsize))) return bail(rc);
+    if ((rc = dalloc(&F.b, c->esz * F.fsize))) return bail(rc);
+    if ((rc = dalloc(&F.k, c->esz * F.fsize))) return bail(rc);
+    if ((rc = dalloc(&F.psi, c->esz * c->npsi * F.fsize))) return bail(rc);
+    if ((rc = dalloc(&F.kpsi, c->esz * c->npsi * F.fsize))) return bail(rc);
+    if (launch_zero(F.psi, c->npsi * F.fsize, c->f32, c->st) != hipSuccess ||
+        launch_zero(F.kpsi, c->npsi * F.fsize, c->f32, c->st) != hipSuccess)
+      return bail(fail(SPARC_E_HIP, "zero fill failed"));
+  }
+  for (int p = 0; p < 4; ++p)
+    if (launch_zero(c->pool[p], (long)c->nv * c->fs, c->f32, c->st) != hipSuccess)
+      return bail(fail(SPARC_E_HIP, "zero fill failed"));
+  if (c->f32)
+    for (int p = 0; p < 2; ++p)
+      if (launch_zero(c->comp[p], (long)c->nv * c->fs, true, c->st) != hipSuccess)
+        return bail(fail(SPARC_E_HIP, "zero fill failed"));
+  if ((rc = upload(c, c->stretch, q.stretch, q.nz))) return bail(rc);
+  if (hipMemcpy(c->stretch_d, q.stretch, sizeof(double) * q.nz, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(SPARC_E_HIP, "stretch upload failed"));
+  if (hipStreamSynchronize(c->st) != hipSuccess) return bail(fail(SPARC_E_HIP, "sync failed"));
+  *out = c;
+  return SPARC_OK;
+}
+
+int sparc_hip_destroy(sparc_ctx* c) {
+  if (!c) return SPARC_OK;
+  (void)hipSetDevice(c->dev);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  free_all(c);
+  delete c;
+  return SPARC_OK;
+}
+
+int sparc_hip_set_background(sparc_ctx* c, const sparc_bg* bg) {
+  if (!c || !bg) return fail(SPARC_E_INVALID, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  const long bgn = c->bg3d ? c->N : c->nz;
+  const double* src[B_COUNT] = {bg->rho0, bg->rhoinv, bg->c2rho0, bg->gradrho0_z, bg->gradp0_z,
+                                bg->spongexyz, bg->p0, bg->box, bg->boy, bg->boz, bg->curlbox,
+                                bg->curlboy, bg->curlboz, bg->gradrho0_x, bg->gradrho0_y,
+                                bg->gradp0_x, bg->gradp0_y, bg->reduction};
+  const int nbg = c->mhd ? B_COUNT : B_P0 + 1;
+  for (int s = 0; s < nbg; ++s) {
+    if (!src[s]) return fail(SPARC_E_INVALID, "missing background array " + std::to_string(s));
+    int rc = upload(c, c->bg[s], src[s], bgn);
+    if (rc) return rc;
+  }
+  if (!bg->g) return fail(SPARC_E_INVALID, "missing g");
+  int rc = upload(c, c->g, bg->g, c->nz);
+  if (rc) return rc;
+  for (int f = 0; f < 6; ++f) {
+    FaceDev& F = c->face[f];
+    if (!F.active) continue;
+    if (!bg->pml_a[f] || !bg->pml_b[f] || !bg->pml_kappa[f])
+      return fail(SPARC_E_INVALID, "missing PML coefficients for an active face");
+    if ((rc = upload(c, F.a, bg->pml_a[f], F.fsize))) return rc;
+    if ((rc = upload(c, F.b, bg->pml_b[f], F.fsize))) return rc;
+    if ((rc = upload(c, F.k, bg->pml_kappa[f], F.fsize))) return rc;
+  }
+  if (c->cfg.pulse_dir) {
+    if (!bg->src_x || !bg->src_y || !bg->src_z) return fail(SPARC_E_INVALID, "missing source profile");
+    if ((rc = upload(c, c->sx, bg->src_x, c->nx))) return rc;
+    if ((rc = upload(c, c->sy, bg->src_y, c->ny))) return rc;
+    if ((rc = upload(c, c->sz, bg->src_z, c->nz))) return rc;
+  }
+  c->dimc = bg->dimc;
+  c->dimrho = bg->dimrho;
+  c->bg_set = true;
+  return SPARC_OK;
+}
+
+int sparc_hip_put_state(sparc_ctx* c, const double* a, const double* const psi[6]) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  if (a) {
+    for (int f = 0; f < c->nv; ++f) {
+      int rc = upload(c, (char*)c->pool[c->ra] + c->esz * (size_t)f * c->fs, a + (long)f * c->N, c->N);
+      if (rc) return rc;
+    }
+  }
+  for (int f = 0; f < 6; ++f) {
+    FaceDev& F = c->face[f];
+    if (!F.active) continue;
+    if (psi && psi[f]) {
+      int rc = upload(c, F.psi, psi[f], c->npsi * F.fsize);
+      if (rc) return rc;
+    }
+    if (launch_zero(F.kpsi, c->npsi * F.fsize, c->f32, c->st) != hipSuccess)
+      return fail(SPARC_E_HIP, "zero fill failed");
+  }
+  if (c->f32)
+    for (int p = 0; p < 2; ++p)
+      if (launch_zero(c->comp[p], (long)c->nv * c->fs, true, c->st) != hipSuccess)
+        return fail(SPARC_E_HIP, "zero fill failed");
+  for (int p : {c->rk0, c->rk1})
+    if (launch_zero(c->pool[p], (long)c->nv * c->fs, c->f32, c->st) != hipSuccess)
+      return fail(SPARC_E_HIP, "zero fill failed");
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->state_set = true;
+  return SPARC_OK;
+}
+
+int sparc_hip_get_state(sparc_ctx* c, double* a, double* const psi[6]) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  if (a) {
+    for (int f = 0; f < c->nv; ++f) {
+      double* dst = a + (long)f * c->N;
+      int rc = download(c, dst, (char*)c->pool[c->ra] + c->esz * (size_t)f * c->fs, c->N);
+      if (rc) return rc;
+      if (c->f32) {  // the compensated value is a - c
+        std::vector<double> cc(c->N);
+        rc = download(c, cc.data(), (char*)c->comp[c->rc] + 4 * (size_t)f * c->fs, c->N);
+        if (rc) return rc;
+        for (long q = 0; q < c->N; ++q) dst[q] -= cc[q];
+      }
+    }
+  }
+  for (int f = 0; f < 6; ++f) {
+    FaceDev& F = c->face[f];
+    if (!F.active || !psi || !psi[f]) continue;
+    int rc = download(c, psi[f], F.psi, c->npsi * F.fsize);
+    if (rc) return rc;
+  }
+  return SPARC_OK;
+}
+
+static void timed_begin(sparc_ctx* c, int kind, TimeRec* r) {
+  if (!c->timing) return;
+  r->kind = kind;
+  (void)hipEventCreate(&r->e0);
+  (void)hipEventCreate(&r->e1);
+  (void)hipEventRecord(r->e0, c->st);
+}
+static void timed_end(sparc_ctx* c, TimeRec* r) {
+  if (!c->timing) return;
+  (void)hipEventRecord(r->e1, c->st);
+  c->trecs.push_back(*r);
+}
+
+template <typename T>
+static int do_step(sparc_ctx* c, double amp) {
+  StageArgs<T> A{};
+  A.nx = c->nx;
+  A.ny = c->ny;
+  A.nz = c->nz;
+  A.kc = c->kc;
+  A.fs = c->fs;
+  A.ntx = c->ntx;
+  A.nty = c->nty;
+  A.nblk_xy = c->ntx * c->nty;
+  A.nchunk = c->nchunk;
+  A.periodic = c->cfg.periodic ? 1 : 0;
+  A.pulse_dir = c->cfg.pulse_dir;
+  A.dt = (T)c->cfg.deltat;
+  A.sxs = (T)c->cfg.stretchx;
+  A.sys = (T)c->cfg.stretchy;
+  A.src_amp = (T)amp;
+  auto B = [&](int s) { return (const T*)c->bg[s]; };
+  A.rho0 = B(B_RHO0);
+  A.rhoinv = B(B_RHOINV);
+  A.c2rho0 = B(B_C2RHO0);
+  A.gr0z = B(B_GR0Z);
+  A.gp0z = B(B_GP0Z);
+  A.sponge = B(B_SPONGE);
+  A.p0 = B(B_P0);
+  if (c->mhd) {
+    A.box = B(B_BOX);
+    A.boy = B(B_BOY);
+    A.boz = B(B_BOZ);
+    A.cbox = B(B_CBOX);
+    A.cboy = B(B_CBOY);
+    A.cboz = B(B_CBOZ);
+    A.gr0x = B(B_GR0X);
+    A.gr0y = B(B_GR0Y);
+    A.gp0x = B(B_GP0X);
+    A.gp0y = B(B_GP0Y);
+    A.red = B(B_RED);
+  }
+  A.g = (const T*)c->g;
+  A.stretch = (const T*)c->stretch;
+  A.sx = (const T*)c->sx;
+  A.sy = (const T*)c->sy;
+  A.sz = (const T*)c->sz;
+  for (int f = 0; f < 6; ++f) {
+    const FaceDev& F = c->face[f];
+    Face<T>& D = A.face[f];
+    D.active = F.active;
+    D.n = F.n;
+    D.off = F.off;
+    D.fsize = F.fsize;
+    D.a = (const T*)F.a;
+    D.b = (const T*)F.b;
+    D.kap = (const T*)F.k;
+    D.psi = (T*)F.psi;
+    D.kpsi = (T*)F.kpsi;
+  }
+  A.partial = c->partial;
+  A.comp = c->f32 ? (T*)c->comp[c->rc] : nullptr;
+  int kin = -1, kout = c->rk0;
+  for (int s = 1; s <= 5; ++s) {
+    A.first = s == 1;
+    A.last = s == 5;
+    A.beta = (T)c->cfg.betas[s - 1];
+    A.a = (const T*)c->pool[c->ra];
+    A.kin = kin >= 0 ? (const T*)c->pool[kin] : nullptr;
+    A.kout = A.last ? nullptr : (T*)c->pool[kout];
+    A.aout = A.last ? (T*)c->pool[c->ra2] : nullptr;
+    TimeRec r;
+    timed_begin(c, A.last ? 1 : 0, &r);
+    hipError_t e = launch_stage<T>(c->mhd, c->bg3d, c->tx, c->ty, A, c->st);
+    timed_end(c, &r);
+    if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("stage launch: ") + hipGetErrorString(e));
+    kin = kout;
+    kout = (kout == c->rk0) ? c->rk1 : c->rk0;
+  }
+  std::swap(c->ra, c->ra2);
+  const int nblk = c->ntx * c->nty * c->nchunk;
+  if (launch_diag_reduce(c->partial, nblk, c->diag, c->st) != hipSuccess)
+    return fail(SPARC_E_HIP, "diag reduce launch failed");
+  c->steps_done++;
+  return SPARC_OK;
+}
+
+int sparc_hip_step(sparc_ctx* c, double amp) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  if (!c->bg_set || !c->state_set) return fail(SPARC_E_STATE, "set_background and put_state must precede step");
+  HIPCHK(hipSetDevice(c->dev));
+  return c->f32 ? do_step<float>(c, amp) : do_step<double>(c, amp);
+}
+
+template <typename T>
+static int filter_pass(sparc_ctx* c, int axis, int g, T s) {
+  TimeRec r;
+  timed_begin(c, 2, &r);
+  hipError_t e = launch_filter<T>(axis, g, (const T*)c->pool[c->ra], (T*)c->pool[c->rk0], c->nx,
+                                  c->ny, c->nz, c->fs, c->nv, c->cfg.periodic, s, 0, c->st);
+  timed_end(c, &r);
+  if (e != hipSuccess) return fail(SPARC_E_HIP, "filter launch failed");
+  std::swap(c->ra, c->rk0);
+  if (c->f32) {  // the compensation term is filtered with the same linear operator
+    e = launch_filter<T>(axis, g, (const T*)c->comp[c->rc], (T*)c->comp[1 - c->rc], c->nx, c->ny,
+                         c->nz, c->fs, c->nv, c->cfg.periodic, s, 0, c->st);
+    if (e != hipSuccess) return fail(SPARC_E_HIP, "filter launch failed");
+    c->rc = 1 - c->rc;
+  }
+  return SPARC_OK;
+}
+
+template <typename T>
+static int do_filter(sparc_ctx* c, int time) {
+  int rc;
+  if (time % c->cfg.xyfreq == 0) {
+    if ((rc = filter_pass<T>(c, 1, c->cfg.g_xy_filt, (T)c->cfg.s_xy))) return rc;
+    if ((rc = filter_pass<T>(c, 0, c->cfg.g_xy_filt, (T)c->cfg.s_xy))) return rc;
+  }
+  if (time % c->cfg.zfreq == 0) {
+    if ((rc = filter_pass<T>(c, 2, c->cfg.g_z_filt, (T)c->cfg.s_z))) return rc;
+  }
+  return SPARC_OK;
+}
+
+int sparc_hip_filter(sparc_ctx* c, int time) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  return c->f32 ? do_filter<float>(c, time) : do_filter<double>(c, time);
+}
+
+int sparc_hip_advance(sparc_ctx* c, int n0, int n1, const double* amp) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  if (n1 < n0) return SPARC_OK;
+  if (c->cfg.pulse_dir && !amp) return fail(SPARC_E_INVALID, "source amplitudes required");
+  for (int nn = n0; nn <= n1; ++nn) {
+    int rc = sparc_hip_step(c, amp ? amp[nn - n0] : 0.0);
+    if (rc) return rc;
+    if (nn != 0 && (rc = sparc_hip_filter(c, nn))) return rc;
+  }
+  return SPARC_OK;
+}
+
+int sparc_hip_diag(sparc_ctx* c, double out[3]) {
+  if (!c || !out) return fail(SPARC_E_INVALID, "null argument");
+  if (c->steps_done == 0) return fail(SPARC_E_STATE, "no step taken yet");
+  HIPCHK(hipSetDevice(c->dev));
+  double h[3];
+  HIPCHK(hipMemcpyAsync(h, c->diag, sizeof h, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const double nrm = std::pow(h[0] / ((double)c->nx * (double)c->ny * (double)c->nz), 0.5);
+  out[0] = nrm * (c->dimc * c->dimc) * c->dimrho;
+  out[1] = h[1];
+  out[2] = h[2];
+  return SPARC_OK;
+}
+
+static int op_common(sparc_ctx* c, const double* in, double* out) {
+  if (!c || !in || !out) return fail(SPARC_E_INVALID, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  if (!c->op_buf) {
+    int rc = dalloc((void**)&c->op_buf, sizeof(double) * 2 * c->N);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(c->op_buf, in, sizeof(double) * c->N, hipMemcpyHostToDevice, c->st));
+  return SPARC_OK;
+}
+
+static int op_finish(sparc_ctx* c, double* out) {
+  HIPCHK(hipMemcpyAsync(out, c->op_buf + c->N, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SPARC_OK;
+}
+
+static int op_deriv(sparc_ctx* c, int axis, const double* in, double* out, int bc) {
+  if (axis < 2 && bc != 1 && bc != 5) return fail(SPARC_E_INVALID, "bc must be 1 or 5");
+  if (axis == 2 && bc != 1) return fail(SPARC_E_INVALID, "z derivatives take bc = 1 (z is never periodic)");
+  int rc = op_common(c, in, out);
+  if (rc) return rc;
+  const double sxy = axis == 0 ? c->cfg.stretchx : c->cfg.stretchy;
+  if (launch_deriv(axis, c->op_buf, c->op_buf + c->N, c->nx, c->ny, c->nz, sxy, c->stretch_d, bc,
+                   c->st) != hipSuccess)
+    return fail(SPARC_E_HIP, "derivative launch failed");
+  return op_finish(c, out);
+}
+
+int sparc_hip_ddx(sparc_ctx* c, const double* in, double* out, int bc) { return op_deriv(c, 0, in, out, bc); }
+int sparc_hip_ddy(sparc_ctx* c, const double* in, double* out, int bc) { return op_deriv(c, 1, in, out, bc); }
+int sparc_hip_ddz(sparc_ctx* c, const double* in, double* out, int bc) { return op_deriv(c, 2, in, out, bc); }
+
+static int op_filt(sparc_ctx* c, int axis, const double* in, double* out) {
+  int rc = op_common(c, in, out);
+  if (rc) return rc;
+  // the bare filtx/filty/filtz operators (FP64): no edge smoothing (that
+  // belongs to FILTER_VARS) and the result is filt itself
+  const int g = axis == 2 ? c->cfg.g_z_filt : c->cfg.g_xy_filt;
+  if (launch_filter<double>(axis, g, c->op_buf, c->op_buf + c->N, c->nx, c->ny, c->nz, c->N, 1,
+                            c->cfg.periodic, 0.0, 1, c->st) != hipSuccess)
+    return fail(SPARC_E_HIP, "filter launch failed");
+  return op_finish(c, out);
+}
+
+int sparc_hip_filtx(sparc_ctx* c, const double* in, double* out) { return op_filt(c, 0, in, out); }
+int sparc_hip_filty(sparc_ctx* c, const double* in, double* out) { return op_filt(c, 1, in, out); }
+int sparc_hip_filtz(sparc_ctx* c, const double* in, double* out) { return op_filt(c, 2, in, out); }
+
+int sparc_hip_sync(sparc_ctx* c) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SPARC_OK;
+}
+
+int sparc_hip_timing(sparc_ctx* c, int enable) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  c->timing = enable != 0;
+  return SPARC_OK;
+}
+
+int sparc_hip_kernel_time(sparc_ctx* c, int kind, long* count, double* total_ms) {
+  if (!c || kind < 0 || kind > 2 || !count || !total_ms) return fail(SPARC_E_INVALID, "bad argument");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  for (auto& r : c->trecs) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r.e0, r.e1));
+    c->tcount[r.kind]++;
+    c->tms[r.kind] += ms;
+    (void)hipEventDestroy(r.e0);
+    (void)hipEventDestroy(r.e1);
+  }
+  c->trecs.clear();
+  *count = c->tcount[kind];
+  *total_ms = c->tms[kind];
+  return SPARC_OK;
+}
+
+int sparc_hip_tile(sparc_ctx* c, int tile[3], long* nblocks) {
+  if (!c || !tile || !nblocks) return fail(SPARC_E_INVALID, "null argument");
+  tile[0] = c->tx;
+  tile[1] = c->ty;
+  tile[2] = c->kc;
+  *nblocks = (long)c->ntx * c->nty * c->nchunk;
+  return SPARC_OK;
+}
+

@@ -1,0 +1,203 @@
+// ops.hip -- digital filters, standalone derivative operators and the
+// diagnostic reduction.
+//
+// Filters (FILTER_VARS, derivatives.f90:659-703): one pass per direction,
+// y then x then z as the reference orders them.  Each pass reads the field,
+// applies the edge smoothing on the fly (Jacobi form: both reference edge
+// statements read the unsmoothed values), filters, and writes a second
+// buffer; the host swaps the buffer roles afterwards.
+//
+// Standalone derivatives (ddx/ddy/ddz, derivatives.f90:10-116,226-341,
+// 451-562) evaluate the 11-point sums in the reference's index order; they
+// serve the background set-up (initialize.f90:993-1016,1219-1228) and the
+// drop-in operator entry points.
+#include "sparc_dev.h"
+
+namespace sparc {
+
+// ---------------------------------------------------------------------------
+// Filters
+// ---------------------------------------------------------------------------
+template <typename T, int AXIS, int G>
+__global__ __launch_bounds__(256) void k_filter(const T* __restrict__ in, T* __restrict__ out,
+                                                int nx, int ny, int nz, long fs, int periodic,
+                                                T s, int bare) {
+  const long N = (long)nx * ny * nz;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int f = blockIdx.y;
+  const int i = (int)(idx % nx);
+  const int j = (int)((idx / nx) % ny);
+  const int k = (int)(idx / ((long)nx * ny));
+  const int n = AXIS == 0 ? nx : AXIS == 1 ? ny : nz;
+  const int pos = AXIS == 0 ? i : AXIS == 1 ? j : k;
+  const long stride = AXIS == 0 ? 1 : AXIS == 1 ? (long)nx : (long)nx * ny;
+  const T* line = in + (long)f * fs + (idx - (long)pos * stride);
+  const bool wrapped = (AXIS < 2) && periodic;
+  // edge smoothing: derivatives.f90:671-675 (y), 680-684 (x), 690-693 (z)
+  // (bare = the filtx/filty/filtz operators alone: no smoothing, result filt)
+  auto sm = [&](int r) -> T {
+    if (!bare && !wrapped && ((r >= 1 && r <= G - 1) || (r >= n - G && r <= n - 2)))
+      return (line[(long)(r - 1) * stride] + line[(long)(r + 1) * stride]) / T(2);
+    return line[(long)r * stride];
+  };
+  T filt;
+  if (wrapped) {
+    T acc = 0;
+#pragma unroll
+    for (int m = -G; m <= G; ++m) acc += line[(long)wrap(pos + m, n) * stride] * filt_w<T>(G, m);
+    filt = acc;
+  } else if (pos < G || pos >= n - G) {
+    // x/y: copy (g = 5) or zero (g = 3); z: copy for both widths
+    // (derivatives.f90:190-192,215-217,414-416,439-441,624,635)
+    filt = (AXIS == 2 || G == 5) ? sm(pos) : T(0);
+  } else {
+    T acc = 0;
+#pragma unroll
+    for (int m = -G; m <= G; ++m) acc += sm(pos + m) * filt_w<T>(G, m);
+    filt = acc;
+  }
+  // g = 5: a = filt; g = 3: a = a - s*filt (derivatives.f90:677-678,686-687,695-696)
+  out[(long)f * fs + idx] = (G == 5 || bare) ? filt : sm(pos) - s * filt;
+}
+
+template <typename T>
+hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
+                         int nfield, int periodic, T s, int bare, hipStream_t st) {
+  const long N = (long)nx * ny * nz;
+  dim3 grid((unsigned)((N + 255) / 256), (unsigned)nfield);
+#define SPARC_F(AX, GG) \
+  hipLaunchKernelGGL((k_filter<T, AX, GG>), grid, dim3(256), 0, st, in, out, nx, ny, nz, fs, periodic, s, bare)
+  if (g == 5) {
+    if (axis == 0) SPARC_F(0, 5);
+    else if (axis == 1) SPARC_F(1, 5);
+    else SPARC_F(2, 5);
+  } else {
+    if (axis == 0) SPARC_F(0, 3);
+    else if (axis == 1) SPARC_F(1, 3);
+    else SPARC_F(2, 3);
+  }
+#undef SPARC_F
+  return hipGetLastError();
+}
+template hipError_t launch_filter<double>(int, int, const double*, double*, int, int, int, long,
+                                          int, int, double, int, hipStream_t);
+template hipError_t launch_filter<float>(int, int, const float*, float*, int, int, int, long, int,
+                                         int, float, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Standalone derivatives (FP64, the reference's summation order)
+// ---------------------------------------------------------------------------
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_deriv(const double* __restrict__ in, double* __restrict__ out,
+                                               int nx, int ny, int nz, double sxy,
+                                               const double* __restrict__ stretch, int bc) {
+  const long N = (long)nx * ny * nz;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int i = (int)(idx % nx);
+  const int j = (int)((idx / nx) % ny);
+  const int k = (int)(idx / ((long)nx * ny));
+  const int n = AXIS == 0 ? nx : AXIS == 1 ? ny : nz;
+  const int pos = AXIS == 0 ? i : AXIS == 1 ? j : k;
+  const long stride = AXIS == 0 ? 1 : AXIS == 1 ? (long)nx : (long)nx * ny;
+  const double* line = in + (idx - (long)pos * stride);
+  const double s = AXIS == 2 ? stretch[k] : sxy;
+  constexpr double cc[11] = {-0.002484594688, +0.020779405824, -0.090320001280, +0.286511173973,
+                             -0.872756993962, 0.0, +0.872756993962, -0.286511173973,
+                             +0.090320001280, -0.020779405824, +0.002484594688};
+  double acc = 0.0;
+  if (AXIS < 2 && bc == 5) {
+    for (int m = 0; m < 11; ++m) acc += line[(long)wrap(pos + m - 5, n) * stride] * cc[m];
+    out[idx] = s * acc;
+  } else if (pos >= n - 5) {
+    const int r = n - 1 - pos;
+    for (int m = 0; m < 11; ++m) acc += line[(long)(n - 1 - m) * stride] * DPB_D[r][m];
+    out[idx] = -s * acc;
+  } else if (AXIS < 2 && pos < 5) {
+    for (int m = 0; m < 11; ++m) acc += line[(long)m * stride] * DPB_D[pos][m];
+    out[idx] = s * acc;
+  } else {
+    // interior; for z this includes the bottom rows over zero ghost cells
+    for (int m = 0; m < 11; ++m) {
+      const int p = pos + m - 5;
+      acc += (p >= 0 ? line[(long)p * stride] : 0.0) * cc[m];
+    }
+    out[idx] = s * acc;
+  }
+}
+
+hipError_t launch_deriv(int axis, const double* in, double* out, int nx, int ny, int nz, double sxy,
+                        const double* stretch, int bc, hipStream_t st) {
+  const long N = (long)nx * ny * nz;
+  dim3 grid((unsigned)((N + 255) / 256));
+  if (axis == 0)
+    hipLaunchKernelGGL(k_deriv<0>, grid, dim3(256), 0, st, in, out, nx, ny, nz, sxy, stretch, bc);
+  else if (axis == 1)
+    hipLaunchKernelGGL(k_deriv<1>, grid, dim3(256), 0, st, in, out, nx, ny, nz, sxy, stretch, bc);
+  else
+    hipLaunchKernelGGL(k_deriv<2>, grid, dim3(256), 0, st, in, out, nx, ny, nz, sxy, stretch, bc);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostic reduction of the per-block partials of the last RK stage.
+// One block, fixed order -> deterministic.  out = {sum m^2, min rho, min p}.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_diag_reduce(const double* __restrict__ partial, int nblk,
+                                                     double* __restrict__ out) {
+  __shared__ double s0[256], s1[256], s2[256];
+  const int t = threadIdx.x;
+  double a = 0.0, r = 1.0e300, p = 1.0e300;
+  for (int b = t; b < nblk; b += 256) {
+    a += partial[3 * (long)b];
+    r = fmin(r, partial[3 * (long)b + 1]);
+    p = fmin(p, partial[3 * (long)b + 2]);
+  }
+  s0[t] = a;
+  s1[t] = r;
+  s2[t] = p;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      s0[t] += s0[t + o];
+      s1[t] = fmin(s1[t], s1[t + o]);
+      s2[t] = fmin(s2[t], s2[t + o]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = s0[0];
+    out[1] = s1[0];
+    out[2] = s2[0];
+  }
+}
+
+hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_reduce, dim3(1), dim3(256), 0, st, partial, nblk, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Small helpers used by the host layer
+// ---------------------------------------------------------------------------
+__global__ void k_fill_zero_d(double* p, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+__global__ void k_fill_zero_f(float* p, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+hipError_t launch_zero(void* p, long n, bool f32, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  dim3 grid((unsigned)((n + 255) / 256));
+  if (f32)
+    hipLaunchKernelGGL(k_fill_zero_f, grid, dim3(256), 0, st, (float*)p, n);
+  else
+    hipLaunchKernelGGL(k_fill_zero_d, grid, dim3(256), 0, st, (double*)p, n);
+  return hipGetLastError();
+}
+
+}  // namespace sparc

@@ -1,0 +1,140 @@
+// sparc_dev.h -- device-side definitions shared by the SPARC2 HIP kernels.
+//
+// Layout in HBM: every field is one contiguous slab of `fs` elements (fs =
+// nx*ny*nz rounded up to a multiple of 256), x fastest, then y, then z -- the
+// reference's Fortran (x, y, z) order, so host <-> device copies are flat.
+// A state buffer holds nv such slabs in the reference's field order
+// (all_modules.f90:104, initialize.f90:780-826): rho, vx, vy, vz, p, bx, by, bz.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sparc {
+
+enum Field { RHO = 0, VX = 1, VY = 2, VZ = 3, P = 4, BX = 5, BY = 6, BZ = 7 };
+enum FaceId { XBOT = 0, XTOP = 1, YBOT = 2, YTOP = 3, ZBOT = 4, ZTOP = 5 };
+
+// ---------------------------------------------------------------------------
+// Coefficient tables (initialize.f90:1927-2034).  The centred 11-point DRP row
+// is antisymmetric with a zero centre, so only its right half is kept:
+// DPC[m-1] multiplies (f(i+m) - f(i-m)), m = 1..5.  DPB[r][m] are the five
+// one-sided rows dpz(r+1, m+1), including the literal corrections the
+// reference adds to some entries.
+// ---------------------------------------------------------------------------
+#define SPARC_DPC {0.872756993962, -0.286511173973, 0.090320001280, \
+                   -0.020779405824, 0.002484594688}
+
+__constant__ static const double DPB_D[5][11] = {
+    {-2.391602219538, 5.832490322294, -7.650218001182 + 0.000000000001,
+     7.907810563576, -5.922599052629, 3.071037015445, -1.014956769726,
+     0.170022256519, 0.002819958377, -0.004791009708, -0.000013063429},
+    {-0.180022054228, -1.237550583044, 2.484731692990, -1.810320814061,
+     1.112990048440, -0.481086916514, 0.126598690230, -0.015510730165,
+     0.000024609059 - 0.000003, 0.000156447571 - 0.000000000001,
+     -0.000007390277},
+    {0.057982271137, -0.536135360383, -0.264089548967 + 0.000000000002,
+     0.917445877606 - 0.000000000002, -0.169688364841, -0.029716326170,
+     0.029681617641, -0.005222483773, -0.000118806260, -0.000118806260,
+     -0.000020069730},
+    {-0.013277273810, 0.115976072920, -0.617479187931,
+     -0.274113948206 + 0.000000000002, 1.086208764655 - 0.000000000002,
+     -0.402951626982, 0.131066986242, -0.028154858354, 0.002596328316,
+     0.000128743150, 0.0},
+    {0.016756572303, -0.117478455239, 0.411034935097, -1.130286765151,
+     0.341435872100 - 0.000000000001, 0.556396830543, -0.082525734207,
+     0.003565834658, 0.001173034777, -0.000071772671 + 0.000000000064,
+     -0.000000352273}};
+
+template <typename T> struct Coef;
+template <> struct Coef<double> {
+  __device__ static double c(int m) {
+    constexpr double v[5] = SPARC_DPC;
+    return v[m];
+  }
+  __device__ static double b(int r, int m) { return DPB_D[r][m]; }
+};
+template <> struct Coef<float> {
+  __device__ static float c(int m) {
+    constexpr double v[5] = SPARC_DPC;
+    return (float)v[m];
+  }
+  __device__ static float b(int r, int m) { return (float)DPB_D[r][m]; }
+};
+
+// Digital filters (initialize.f90:1932-1951): the 11-point filter is
+// 1 - sin^10(theta/2) (binomial(10,k)/1024), the 7-point one sin^6(theta/2).
+// Both are symmetric; W(g, m) is the weight at offset m in [-g, g].
+template <typename T> __device__ inline T filt_w(int g, int m) {
+  constexpr double w11[6] = {0.75390625000000, 0.5 * 0.41015625000000,
+                             -0.5 * 0.23437500000000, 0.5 * 0.08789062500000,
+                             -0.5 * 0.01953125000000, 0.5 * 0.00195312500000};
+  constexpr double w7[4] = {0.312500000, -0.234375000, 0.093750000, -0.015625000};
+  int am = m < 0 ? -m : m;
+  return (T)(g == 5 ? w11[am] : w7[am]);
+}
+
+// ---------------------------------------------------------------------------
+// One CPML face (RHS.f90:50-93, 424-546; storage initialize.f90:541-585).
+// Face arrays are (n, ny, nz) for x faces, (nx, n, nz) for y faces and
+// (nx, ny, n) for z faces, x fastest.  psi holds the RK base values
+// (zpmlvars & co.), kpsi the stage right-hand sides (scrzpml & co.), each
+// npsi slabs of `fsize`.
+// ---------------------------------------------------------------------------
+template <typename T> struct Face {
+  int active;
+  int n;       // width along the face's axis
+  int off;     // first global index along that axis
+  int pad_;
+  long fsize;
+  const T* a;
+  const T* b;
+  const T* kap;
+  T* psi;
+  T* kpsi;
+};
+
+template <typename T> struct StageArgs {
+  int nx, ny, nz, kc;
+  long fs;
+  int ntx, nty, nblk_xy, nchunk;
+  int periodic, first, last, pulse_dir;
+  T beta, dt, sxs, sys, src_amp;
+  const T* a;
+  const T* kin;
+  T* kout;
+  T* aout;
+  T* comp;  // Kahan compensation (fp32 only), updated in place on the last stage
+  // backgrounds: 3-D (bg3d) or nz-long z profiles
+  const T *rho0, *rhoinv, *c2rho0, *gr0z, *gp0z, *sponge, *p0;
+  // MHD backgrounds (always 3-D)
+  const T *box, *boy, *boz, *cbox, *cboy, *cboz, *gr0x, *gr0y, *gp0x, *gp0y, *red;
+  const T *g, *stretch, *sx, *sy, *sz;
+  Face<T> face[6];
+  double* partial;  // last stage: 3 doubles per block (KE sum, min rho, min p)
+};
+
+// PML update of one derivative family at one point (RHS.f90:52-55): the psi
+// RHS comes from the raw derivative, then the derivative is replaced by
+// d/kappa + psi.  On the last stage the RK base is advanced in place
+// (step.f90:59-61); otherwise the stage RHS is stored in place.
+template <typename T>
+__device__ __forceinline__ void pml_family(const Face<T>& F, long fp, int q, T& d,
+                                           bool first, bool last, T beta, T dt) {
+  const long o = (long)q * F.fsize + fp;
+  const T base = F.psi[o];
+  const T ps = first ? base : base + beta * F.kpsi[o];
+  const T kn = F.a[fp] * d + F.b[fp] * ps;
+  d = d / F.kap[fp] + ps;
+  if (last)
+    F.psi[o] = base + dt * kn;
+  else
+    F.kpsi[o] = kn;
+}
+
+__device__ __forceinline__ int wrap(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+
+}  // namespace sparc

@@ -1,0 +1,219 @@
+"""Solver: one SPARC2 run resident on one GPU, driven through the C ABI.
+
+Method names mirror the reference routines they replace:
+
+    Solver.step(time)         STEP (step.f90:20-63), source at t = time*timestep
+    Solver.filter_vars(time)  FILTER_VARS (derivatives.f90:659-703)
+    Solver.ddx/ddy/ddz        ddx/ddy/ddz (derivatives.f90:10,226,451)
+    Solver.filtx/filty/filtz  filtx/filty/filtz (derivatives.f90:119,345,565)
+    Solver.norm_ke / diag     norm() + the negativity guards (driver.f90:161-183)
+    Solver.run(n0, n1)        the driver loop body (driver.f90:123-200)
+
+State arrays are numpy (nv, nz, ny, nx) FP64 == the reference's Fortran
+``a(nx, ny, nz, nv)``.  PML bases are per face, (npsi, *face_shape).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import SparcBg, SparcCfg, check, dptr, lib
+from .config import RunConfig
+from .initialise import (SetupState, initialise_rhs, layout_for_device, prepare,
+                         source_amplitude, source_profiles)
+
+
+class DivergenceError(RuntimeError):
+    """Raised by ``run`` when the reference's failure guards trip (driver.f90:175-183)."""
+
+
+class Solver:
+    def __init__(self, cfg: RunConfig, raw_bg: Dict[str, np.ndarray], device: int = 0,
+                 tile: Sequence[int] = (0, 0, 0), setup: Optional[SetupState] = None):
+        self.cfg = cfg
+        self.st = setup if setup is not None else prepare(cfg, raw_bg)
+        st = self.st
+        L = lib()
+        self._keep = []
+        c = SparcCfg()
+        c.nx, c.ny, c.nz = cfg.nx, cfg.ny, cfg.nz
+        c.option = 3 if cfg.mhd else 1
+        c.precision = _lib.PREC_FP32 if cfg.precision == "fp32" else _lib.PREC_FP64
+        c.periodic = int(cfg.periodic)
+        c.use_pml = int(cfg.use_pml)
+        c.use_hpml = int(cfg.use_hpml)
+        for f in range(6):
+            c.npml[f] = cfg.npml[f]
+        c.g_xy_filt, c.g_z_filt = cfg.g_xy_filt, cfg.g_z_filt
+        c.s_xy, c.s_z = cfg.s_xy, cfg.s_z
+        c.xyfreq, c.zfreq = cfg.xyfreq, cfg.zfreq
+        c.pulse_dir = cfg.pulse_dir if cfg.pulse else 0
+        c.bg3d = int(st.bg3d)
+        c.deltat = st.deltat
+        for s in range(5):
+            c.betas[s] = float(st.betas[s])
+        c.stretchx, c.stretchy = st.stretchx, st.stretchy
+        self._stretch = np.ascontiguousarray(st.stretch, dtype=np.float64)
+        c.stretch = dptr(self._stretch)
+        c.device = device
+        for q in range(3):
+            c.tile[q] = int(tile[q])
+        self._cfg = c
+        h = C.c_void_p()
+        check(L.sparc_hip_create(C.byref(c), C.byref(h)), "sparc_hip_create")
+        self.h = h
+        self.nv = cfg.nv
+        self.npsi = cfg.npsi
+        self.shape = (cfg.nz, cfg.ny, cfg.nx)
+        if setup is None:
+            initialise_rhs(st, self)
+        self._upload_background()
+        self.put_state(np.zeros((self.nv,) + self.shape))
+
+    # ---------------------------------------------------------------- set-up
+    def _upload_background(self):
+        st = self.st
+        dev = layout_for_device(st)
+        b = SparcBg()
+        for name, v in dev.items():
+            setattr(b, name, dptr(v))
+            self._keep.append(v)
+        for f in range(6):
+            if f in st.pml:
+                for key, fld in (("a", "pml_a"), ("b", "pml_b"), ("kappa", "pml_kappa")):
+                    v = np.ascontiguousarray(st.pml[f][key], dtype=np.float64)
+                    self._keep.append(v)
+                    getattr(b, fld)[f] = dptr(v)
+        if self.cfg.pulse:
+            sx, sy, sz = (np.ascontiguousarray(v) for v in source_profiles(st))
+            self._keep += [sx, sy, sz]
+            b.src_x, b.src_y, b.src_z = dptr(sx), dptr(sy), dptr(sz)
+        b.dimc, b.dimrho = st.dimc, st.dimrho
+        check(lib().sparc_hip_set_background(self.h, C.byref(b)), "sparc_hip_set_background")
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib().sparc_hip_destroy(h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ----------------------------------------------------------------- state
+    def face_shape(self, f):
+        return (self.npsi,) + self.cfg.face_shape(f)
+
+    def put_state(self, a, psi: Optional[Dict[int, np.ndarray]] = None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (self.nv,) + self.shape:
+            raise ValueError(f"state shape {a.shape} != {(self.nv,) + self.shape}")
+        ps = (_lib.DP * 6)()
+        keep = [a]
+        for f in range(6):
+            if psi is not None and f in psi and self.cfg.face_active(f):
+                v = np.ascontiguousarray(psi[f], dtype=np.float64)
+                if v.size != int(np.prod(self.face_shape(f))):
+                    raise ValueError(f"psi face {f} has {v.size} values")
+                keep.append(v)
+                ps[f] = dptr(v)
+        check(lib().sparc_hip_put_state(self.h, dptr(a), ps), "sparc_hip_put_state")
+
+    def get_state(self, with_psi: bool = False):
+        a = np.zeros((self.nv,) + self.shape)
+        ps = (_lib.DP * 6)()
+        out = {}
+        if with_psi:
+            for f in range(6):
+                if self.cfg.face_active(f):
+                    out[f] = np.zeros(self.face_shape(f))
+                    ps[f] = dptr(out[f])
+        check(lib().sparc_hip_get_state(self.h, dptr(a), ps), "sparc_hip_get_state")
+        return (a, out) if with_psi else a
+
+    # ------------------------------------------------------------- time step
+    def amplitude(self, time: int) -> float:
+        return source_amplitude(self.st, time)
+
+    def step(self, time: int):
+        check(lib().sparc_hip_step(self.h, self.amplitude(time)), "sparc_hip_step")
+
+    def filter_vars(self, time: int):
+        check(lib().sparc_hip_filter(self.h, int(time)), "sparc_hip_filter")
+
+    def advance(self, n0: int, n1: int):
+        amps = np.array([self.amplitude(t) for t in range(n0, n1 + 1)], dtype=np.float64)
+        check(lib().sparc_hip_advance(self.h, int(n0), int(n1), dptr(amps)), "sparc_hip_advance")
+
+    def diag(self) -> np.ndarray:
+        out = np.zeros(3)
+        check(lib().sparc_hip_diag(self.h, dptr(out)), "sparc_hip_diag")
+        return out
+
+    def run(self, n0: int, n1: int, check_every: int = 1) -> np.ndarray:
+        """Driver loop nn = n0..n1 with the reference's failure guards
+        evaluated every ``check_every`` steps (driver.f90:175-196)."""
+        hist = []
+        for nn in range(n0, n1 + 1):
+            self.step(nn)
+            if (nn - n0) % check_every == 0 or nn == n1:
+                d = self.diag()
+                hist.append(d)
+                if not (d[0] <= 1.0e3) or d[1] <= 0.0 or d[2] <= 0.0:
+                    raise DivergenceError(f"step {nn}: norm_KE={d[0]:.5e} min(rho+rho0)={d[1]:.5e} "
+                                          f"min(p+p0)={d[2]:.5e}")
+            if nn != 0:
+                self.filter_vars(nn)
+        return np.array(hist)
+
+    def sync(self):
+        check(lib().sparc_hip_sync(self.h), "sparc_hip_sync")
+
+    # ------------------------------------------------------------- operators
+    def _op(self, fn, x, *extra):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != int(np.prod(self.shape)):
+            raise ValueError("operator input must hold nx*ny*nz values")
+        out = np.zeros(self.shape)
+        check(fn(self.h, dptr(x), dptr(out), *extra), fn.__name__)
+        return out
+
+    def ddx(self, x, bc):
+        return self._op(lib().sparc_hip_ddx, x, int(bc))
+
+    def ddy(self, x, bc):
+        return self._op(lib().sparc_hip_ddy, x, int(bc))
+
+    def ddz(self, x, bc=1):
+        return self._op(lib().sparc_hip_ddz, x, int(bc))
+
+    def filtx(self, x):
+        return self._op(lib().sparc_hip_filtx, x)
+
+    def filty(self, x):
+        return self._op(lib().sparc_hip_filty, x)
+
+    def filtz(self, x):
+        return self._op(lib().sparc_hip_filtz, x)
+
+    # ----------------------------------------------------------- measurement
+    def timing(self, enable: bool):
+        check(lib().sparc_hip_timing(self.h, int(enable)), "sparc_hip_timing")
+
+    def kernel_time(self, kind: int):
+        n = C.c_long()
+        ms = C.c_double()
+        check(lib().sparc_hip_kernel_time(self.h, kind, C.byref(n), C.byref(ms)), "kernel_time")
+        return n.value, ms.value
+
+    def tile(self):
+        t = (C.c_int * 3)()
+        nb = C.c_long()
+        check(lib().sparc_hip_tile(self.h, t, C.byref(nb)), "sparc_hip_tile")
+        return tuple(t), nb.value
