@@ -93,8 +93,8 @@ struct sparc_ctx {
   double* op_buf = nullptr;  // 2*N doubles for the standalone operators
   bool timing = false;
   std::vector<TimeRec> trecs;
-  long tcount[3] = {0, 0, 0};
-  double tms[3] = {0, 0, 0};
+  long tcount[4] = {0, 0, 0, 0};
+  double tms[4] = {0, 0, 0, 0};
 };
 
 static long round_up(long v, long m) { return (v + m - 1) / m * m; }
@@ -526,7 +526,7 @@ static int do_step(sparc_ctx* c, double amp) {
     A.kout = A.last ? nullptr : (T*)c->pool[kout];
     A.aout = A.last ? (T*)c->pool[c->ra2] : nullptr;
     TimeRec r;
-    timed_begin(c, A.last ? 1 : 0, &r);
+    timed_begin(c, A.first ? 0 : (A.last ? 2 : 1), &r);
     hipError_t e = launch_stage<T>(c->mhd, c->bg3d, c->tx, c->ty, A, c->st);
     timed_end(c, &r);
     if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("stage launch: ") + hipGetErrorString(e));
@@ -551,7 +551,7 @@ int sparc_hip_step(sparc_ctx* c, double amp) {
 template <typename T>
 static int filter_pass(sparc_ctx* c, int axis, int g, T s) {
   TimeRec r;
-  timed_begin(c, 2, &r);
+  timed_begin(c, 3, &r);
   hipError_t e = launch_filter<T>(axis, g, (const T*)c->pool[c->ra], (T*)c->pool[c->rk0], c->nx,
                                   c->ny, c->nz, c->fs, c->nv, c->cfg.periodic, s, 0, c->st);
   timed_end(c, &r);
@@ -674,7 +674,7 @@ int sparc_hip_timing(sparc_ctx* c, int enable) {
 }
 
 int sparc_hip_kernel_time(sparc_ctx* c, int kind, long* count, double* total_ms) {
-  if (!c || kind < 0 || kind > 2 || !count || !total_ms) return fail(SPARC_E_INVALID, "bad argument");
+  if (!c || kind < 0 || kind > 3 || !count || !total_ms) return fail(SPARC_E_INVALID, "bad argument");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   for (auto& r : c->trecs) {

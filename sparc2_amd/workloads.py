@@ -1,0 +1,70 @@
+"""The benchmark workloads, built with the choices the reference leaves open.
+
+* ``quiet_acoustic``  -- plain quiet-Sun acoustic run: 64 x 64 x 128, 50 Mm
+  periodic, z in [-10, +0.5] Mm, polytrope background, vertical pulse on vz at
+  the domain centre and z = -0.15 Mm (sigma 1 Mm), 3 mHz carrier
+  (t0 = 333.33 s), 11-point stencils, z filter every 10 s, xy filter every
+  30 s, 10-cell CPML top and bottom, dt = 1 s, 500 steps.
+* ``sunspot_mhd`` -- plain sunspot MHD run: 256^3, 100 Mm, z in [-20, +1] Mm,
+  flux tube (3 kG on axis, Gaussian radius 4 Mm at z = 0), Alfven speed capped
+  at 100 km/s, CPML on all six faces (N = 15, RC = ln(1e4) = 9.2103, n = 2,
+  kappa = 1, sponge 0.1), pulse 10 Mm off axis, dt = 0.5 s, 2000 steps.
+* ``quiet_precision`` -- quiet_acoustic's physics at 512 x 512 x 300.
+
+Reference facts these follow: the pulse is sin x Gaussian (driver.f90:140-145;
+there is no Ricker option), xy stencils are 11-point like z
+(all_modules.f90:153, initialize.f90:2031), the PML line is
+"N, RC, n, kappa, sponge" with RC used as -ln(R) (initialize.f90:1055), and
+USE_HPML switches periodicity off (initialize.f90:1610-1620).
+"""
+from __future__ import annotations
+
+from .background import make_background
+from .config import RunConfig
+
+PML_Z = (9.2103, 2.0, 1.0, 0.1)
+
+
+def quiet_acoustic(nx=64, ny=64, nz=128, precision="fp64", **kw):
+    cfg = RunConfig(
+        nx=nx, ny=ny, nz=nz, xlength=50.0e8, ylength=50.0e8, timestep=1.0,
+        solartime=500.0 / 3600.0, magnetic=0, periodic=True, use_pml=True, use_hpml=False,
+        stabilise=True, npml=(0, 0, 0, 0, 10, 10), pmlc=(PML_Z,) * 6,
+        filter_mode_xy=1, filter_mode_z=1, t_xy=30.0, t_z=10.0,
+        pulse=True, pulse_dir=3, pulse_t0=1000.0 / 3.0, pulse_t1=450.0, pulse_st=150.0,
+        pulse_px=25.0e8, pulse_py=25.0e8, pulse_pz=-0.15e8, pulse_sr=1.0e8, pulse_sz=1.0e8,
+        pulse_amp=1.0e4, precision=precision)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg, (-10.0, 0.5)
+
+
+def sunspot_mhd(nx=256, ny=256, nz=256, precision="fp64", **kw):
+    pml = (9.2103, 2.0, 1.0, 0.1)
+    cfg = RunConfig(
+        nx=nx, ny=ny, nz=nz, xlength=100.0e8, ylength=100.0e8, timestep=0.5,
+        solartime=1000.0 / 3600.0, magnetic=2, periodic=False, use_pml=True, use_hpml=True,
+        stabilise=True, npml=(15, 15, 15, 15, 15, 15), pmlc=(pml,) * 6,
+        filter_mode_xy=1, filter_mode_z=1, t_xy=30.0, t_z=10.0,
+        pulse=True, pulse_dir=3, pulse_t0=1000.0 / 3.0, pulse_t1=450.0, pulse_st=150.0,
+        pulse_px=60.0e8, pulse_py=50.0e8, pulse_pz=-0.15e8, pulse_sr=1.0e8, pulse_sz=1.0e8,
+        pulse_amp=1.0e4, maxva=1.0e7, precision=precision)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg, (-20.0, 1.0)
+
+
+def quiet_precision(precision="fp32", **kw):
+    return quiet_acoustic(nx=512, ny=512, nz=300, precision=precision, **kw)
+
+
+def build(name: str, **kw):
+    """(cfg, raw background) of a named workload."""
+    fn = {"quiet_acoustic": quiet_acoustic, "sunspot_mhd": sunspot_mhd,
+          "quiet_precision": quiet_precision}[name]
+    cfg, (zb, zt) = fn(**kw)
+    if cfg.magnetic == 2:
+        bg = make_background(cfg, zb, zt, B0=3000.0, R0=4.0e8)
+    else:
+        bg = make_background(cfg, zb, zt)
+    return cfg, bg
