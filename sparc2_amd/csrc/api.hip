@@ -2,8 +2,9 @@
 //
 // A context owns every device buffer the reference's STEP / FILTER_VARS read
 // or write (module arrays of all_modules.f90:96-146).  State-sized buffers
-// form a pool of four (base state, its ping-pong partner, two stage RHS
-// buffers); their roles are swapped instead of copied.
+// form a pool of three: the RK base a and two stage-value buffers that the
+// stages ping-pong between (a filter pass uses one of them as its output and
+// swaps roles with a instead of copying back).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,7 +57,7 @@ enum BgSlot {
 struct FaceDev {
   int active = 0, n = 0, off = 0;
   long fsize = 0;
-  void *a = nullptr, *b = nullptr, *k = nullptr, *psi = nullptr, *kpsi = nullptr;
+  void *a = nullptr, *b = nullptr, *k = nullptr, *psi = nullptr, *pst = nullptr;
 };
 
 struct TimeRec {
@@ -76,8 +77,8 @@ struct sparc_ctx {
   int nx = 0, ny = 0, nz = 0;
   long N = 0, fs = 0;
   size_t esz = 8;
-  void* pool[4] = {nullptr, nullptr, nullptr, nullptr};
-  int ra = 0, ra2 = 1, rk0 = 2, rk1 = 3;
+  void* pool[3] = {nullptr, nullptr, nullptr};
+  int ra = 0, rk0 = 1, rk1 = 2;
   void* comp[2] = {nullptr, nullptr};
   int rc = 0;
   void* bg[B_COUNT] = {};
@@ -160,7 +161,7 @@ static void free_all(sparc_ctx* c) {
     F(f.b);
     F(f.k);
     F(f.psi);
-    F(f.kpsi);
+    F(f.pst);
   }
   for (auto& r : c->trecs) {
     (void)hipEventDestroy(r.e0);
@@ -289,7 +290,7 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
     return bail(fail(SPARC_E_HIP, "hipStreamCreate failed"));
 
   const size_t sbytes = c->esz * (size_t)c->nv * (size_t)c->fs;
-  for (int p = 0; p < 4; ++p)
+  for (int p = 0; p < 3; ++p)
     if ((rc = dalloc(&c->pool[p], sbytes))) return bail(rc);
   if (c->f32)
     for (int p = 0; p < 2; ++p)
@@ -321,12 +322,12 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
     if ((rc = dalloc(&F.b, c->esz * F.fsize))) return bail(rc);
     if ((rc = dalloc(&F.k, c->esz * F.fsize))) return bail(rc);
     if ((rc = dalloc(&F.psi, c->esz * c->npsi * F.fsize))) return bail(rc);
-    if ((rc = dalloc(&F.kpsi, c->esz * c->npsi * F.fsize))) return bail(rc);
+    if ((rc = dalloc(&F.pst, c->esz * c->npsi * F.fsize))) return bail(rc);
     if (launch_zero(F.psi, c->npsi * F.fsize, c->f32, c->st) != hipSuccess ||
-        launch_zero(F.kpsi, c->npsi * F.fsize, c->f32, c->st) != hipSuccess)
+        launch_zero(F.pst, c->npsi * F.fsize, c->f32, c->st) != hipSuccess)
       return bail(fail(SPARC_E_HIP, "zero fill failed"));
   }
-  for (int p = 0; p < 4; ++p)
+  for (int p = 0; p < 3; ++p)
     if (launch_zero(c->pool[p], (long)c->nv * c->fs, c->f32, c->st) != hipSuccess)
       return bail(fail(SPARC_E_HIP, "zero fill failed"));
   if (c->f32)
@@ -374,7 +375,9 @@ int sparc_hip_set_background(sparc_ctx* c, const sparc_bg* bg) {
       return fail(SPARC_E_INVALID, "missing PML coefficients for an active face");
     if ((rc = upload(c, F.a, bg->pml_a[f], F.fsize))) return rc;
     if ((rc = upload(c, F.b, bg->pml_b[f], F.fsize))) return rc;
-    if ((rc = upload(c, F.k, bg->pml_kappa[f], F.fsize))) return rc;
+    std::vector<double> kinv(F.fsize);  // the stage kernel multiplies by 1/kappa
+    for (long q = 0; q < F.fsize; ++q) kinv[q] = 1.0 / bg->pml_kappa[f][q];
+    if ((rc = upload(c, F.k, kinv.data(), F.fsize))) return rc;
   }
   if (c->cfg.pulse_dir) {
     if (!bg->src_x || !bg->src_y || !bg->src_z) return fail(SPARC_E_INVALID, "missing source profile");
@@ -404,7 +407,7 @@ int sparc_hip_put_state(sparc_ctx* c, const double* a, const double* const psi[6
       int rc = upload(c, F.psi, psi[f], c->npsi * F.fsize);
       if (rc) return rc;
     }
-    if (launch_zero(F.kpsi, c->npsi * F.fsize, c->f32, c->st) != hipSuccess)
+    if (launch_zero(F.pst, c->npsi * F.fsize, c->f32, c->st) != hipSuccess)
       return fail(SPARC_E_HIP, "zero fill failed");
   }
   if (c->f32)
@@ -510,30 +513,29 @@ static int do_step(sparc_ctx* c, double amp) {
     D.fsize = F.fsize;
     D.a = (const T*)F.a;
     D.b = (const T*)F.b;
-    D.kap = (const T*)F.k;
+    D.kinv = (const T*)F.k;
     D.psi = (T*)F.psi;
-    D.kpsi = (T*)F.kpsi;
+    D.pst = (T*)F.pst;
   }
   A.partial = c->partial;
   A.comp = c->f32 ? (T*)c->comp[c->rc] : nullptr;
-  int kin = -1, kout = c->rk0;
+  // u_1 = a; stage s reads u_s and writes u_{s+1}; the last stage updates a
+  int uin = c->ra, uout = c->rk0;
+  A.a = (T*)c->pool[c->ra];
   for (int s = 1; s <= 5; ++s) {
     A.first = s == 1;
     A.last = s == 5;
-    A.beta = (T)c->cfg.betas[s - 1];
-    A.a = (const T*)c->pool[c->ra];
-    A.kin = kin >= 0 ? (const T*)c->pool[kin] : nullptr;
-    A.kout = A.last ? nullptr : (T*)c->pool[kout];
-    A.aout = A.last ? (T*)c->pool[c->ra2] : nullptr;
+    A.beta_next = A.last ? (T)0 : (T)c->cfg.betas[s];
+    A.u = (const T*)c->pool[uin];
+    A.un = A.last ? nullptr : (T*)c->pool[uout];
     TimeRec r;
     timed_begin(c, A.first ? 0 : (A.last ? 2 : 1), &r);
     hipError_t e = launch_stage<T>(c->mhd, c->bg3d, c->tx, c->ty, A, c->st);
     timed_end(c, &r);
     if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("stage launch: ") + hipGetErrorString(e));
-    kin = kout;
-    kout = (kout == c->rk0) ? c->rk1 : c->rk0;
+    uin = uout;
+    uout = (uout == c->rk0) ? c->rk1 : c->rk0;
   }
-  std::swap(c->ra, c->ra2);
   const int nblk = c->ntx * c->nty * c->nchunk;
   if (launch_diag_reduce(c->partial, nblk, c->diag, c->st) != hipSuccess)
     return fail(SPARC_E_HIP, "diag reduce launch failed");

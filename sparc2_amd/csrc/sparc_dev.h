@@ -78,8 +78,8 @@ template <typename T> __device__ inline T filt_w(int g, int m) {
 // One CPML face (RHS.f90:50-93, 424-546; storage initialize.f90:541-585).
 // Face arrays are (n, ny, nz) for x faces, (nx, n, nz) for y faces and
 // (nx, ny, n) for z faces, x fastest.  psi holds the RK base values
-// (zpmlvars & co.), kpsi the stage right-hand sides (scrzpml & co.), each
-// npsi slabs of `fsize`.
+// (zpmlvars & co.), pst the stage values (psizpml & co.), each npsi slabs of
+// `fsize`; kinv is 1/kappa.
 // ---------------------------------------------------------------------------
 template <typename T> struct Face {
   int active;
@@ -89,21 +89,24 @@ template <typename T> struct Face {
   long fsize;
   const T* a;
   const T* b;
-  const T* kap;
+  const T* kinv;
   T* psi;
-  T* kpsi;
+  T* pst;
 };
 
+// One RK stage.  u is the stage value (read with stencil halos), a the RK
+// base (read at the point only).  A middle stage writes the next stage value
+// a + beta_next*K into un; the last stage overwrites a with a + dt*K
+// (step.f90:58) -- safe in place because a is only read at the point.
 template <typename T> struct StageArgs {
   int nx, ny, nz, kc;
   long fs;
   int ntx, nty, nblk_xy, nchunk;
   int periodic, first, last, pulse_dir;
-  T beta, dt, sxs, sys, src_amp;
-  const T* a;
-  const T* kin;
-  T* kout;
-  T* aout;
+  T beta_next, dt, sxs, sys, src_amp;
+  const T* u;
+  T* a;
+  T* un;
   T* comp;  // Kahan compensation (fp32 only), updated in place on the last stage
   // backgrounds: 3-D (bg3d) or nz-long z profiles
   const T *rho0, *rhoinv, *c2rho0, *gr0z, *gp0z, *sponge, *p0;
@@ -116,20 +119,56 @@ template <typename T> struct StageArgs {
 
 // PML update of one derivative family at one point (RHS.f90:52-55): the psi
 // RHS comes from the raw derivative, then the derivative is replaced by
-// d/kappa + psi.  On the last stage the RK base is advanced in place
-// (step.f90:59-61); otherwise the stage RHS is stored in place.
+// d/kappa + psi.  The next psi stage value (step.f90:34-36) or, on the last
+// stage, the advanced base (step.f90:59-61) is written in place.
+template <typename T> struct Buf;
 template <typename T>
-__device__ __forceinline__ void pml_family(const Face<T>& F, long fp, int q, T& d,
-                                           bool first, bool last, T beta, T dt) {
-  const long o = (long)q * F.fsize + fp;
-  const T base = F.psi[o];
-  const T ps = first ? base : base + beta * F.kpsi[o];
-  const T kn = F.a[fp] * d + F.b[fp] * ps;
-  d = d / F.kap[fp] + ps;
+__device__ __forceinline__ void pml_family(const Face<T>& F, uint32_t fp, int q, T& d,
+                                           bool first, bool last, T beta_next, T dt);
+
+// ---------------------------------------------------------------------------
+// Raw buffer view of one array: a uniform descriptor (SGPRs) plus a 32-bit
+// per-lane byte offset, so every array indexed the same way shares one VGPR
+// offset instead of a 64-bit address per array.  Loads past `n` elements
+// return 0 and stores past it are dropped (the descriptor's range check).
+// dword3 = 0x00020000 is the raw-buffer format word of the gfx9 family.
+// ---------------------------------------------------------------------------
+template <typename T> struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Buf(const T* p, uint32_t n)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * (uint32_t)sizeof(T)),
+                                            0x00020000)) {}
+  __device__ __forceinline__ T operator[](uint32_t i) const {
+    if constexpr (sizeof(T) == 8) {
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(i * 8u), 0, 0));
+    } else {
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, 0));
+    }
+  }
+  __device__ __forceinline__ void st(uint32_t i, T v) const {
+    if constexpr (sizeof(T) == 8) {
+      using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, (int)(i * 8u), 0, 0);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)(i * 4u), 0, 0);
+    }
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ void pml_family(const Face<T>& F, uint32_t fp, int q, T& d,
+                                           bool first, bool last, T beta_next, T dt) {
+  const uint32_t n = (uint32_t)F.fsize;
+  const Buf<T> psi(F.psi + (size_t)q * n, n), pst(F.pst + (size_t)q * n, n);
+  const Buf<T> fa(F.a, n), fb(F.b, n), ki(F.kinv, n);
+  const T base = psi[fp];
+  const T ps = first ? base : pst[fp];
+  const T kn = fa[fp] * d + fb[fp] * ps;
+  d = d * ki[fp] + ps;
   if (last)
-    F.psi[o] = base + dt * kn;
+    psi.st(fp, base + dt * kn);
   else
-    F.kpsi[o] = kn;
+    pst.st(fp, base + beta_next * kn);
 }
 
 __device__ __forceinline__ int wrap(int i, int n) {

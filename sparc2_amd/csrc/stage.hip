@@ -1,44 +1,58 @@
 // stage.hip -- one fused low-storage RK stage of the SPARC2 right-hand side.
 //
-// One launch computes, for every grid point,
-//   u   = a + beta_s K_{s-1}            (stage value, step.f90:27-37)
-//   K_s = RHS(u)                        (MP_QUIET_3D RHS.f90:18-175 or
-//                                        MP_MHD_3D RHS.f90:363-668)
-// including the 11-point derivatives (derivatives.f90 ddx/ddy/ddz), the CPML
-// auxiliary updates on all six faces, the pulse source and the zeroing of the
-// boundary planes.  The last stage writes a + dt K_5 (step.f90:58-61) instead
-// of K_5, advances the CPML bases in place and leaves per-block diagnostic
-// partials (driver.f90:161,180).
+// The LDDRK-5 step of step.f90:20-63 is
+//   u_1 = a;  K_s = RHS(u_s);  u_{s+1} = a + beta_{s+1} K_s;  a <- a + dt K_5.
+// One launch of k_stage computes K_s for every grid point -- the 11-point
+// derivatives (derivatives.f90 ddx/ddy/ddz), the CPML auxiliary updates on
+// all six faces, the pulse source and the boundary-plane zeroing of
+// MP_QUIET_3D (RHS.f90:18-175) or MP_MHD_3D (RHS.f90:363-668) -- and
+// immediately writes the next stage value u_{s+1} (or, on the last stage,
+// the advanced base a and per-block diagnostic partials).  Stencils therefore
+// read a single stage-value array; the base a is read at the point only.
 //
 // Blocking (2.5-D): a block owns a TX x TY column tile and marches up a chunk
 // of z planes.  Fields differentiated in z live in an 11-deep register queue
-// per thread (each plane is loaded once); fields differentiated in x or y are
-// staged plane by plane through LDS with 5-cell halos (periodic wrap at the
-// load).  Boundary rows use the reference's one-sided closures.
+// per thread; fields differentiated in x or y are staged plane by plane into
+// LDS with 5-cell halos in one load phase per plane (each staged cell loads
+// every field it feeds once).  The staging addresses, including the periodic
+// wrap, are computed once per block.  Boundary rows use the reference's
+// one-sided closures.
 #include "sparc_dev.h"
 
 namespace sparc {
 
-enum Kind { K_RHO = 0, K_VX, K_VY, K_VZ, K_P, K_BX, K_BY, K_BZ, K_F1, K_F2, K_F3 };
+// Occupancy target (waves per SIMD): the register queue of MHD in FP64
+// (6 fields x 11 planes) needs about 200 registers, so it runs 2 waves per
+// SIMD; the other variants are held to 3 so memory latency overlaps.
+template <typename T, bool MHD>
+constexpr int stage_waves() {
+  return MHD ? (sizeof(T) == 8 ? 2 : 3) : 3;
+}
 
 template <typename T, bool MHD, bool BG3D, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_stage(const StageArgs<T> A) {
+__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage(const StageArgs<T> A) {
   constexpr int NT = TX * TY;
   constexpr int LX = TX + 10, LY = TY + 10;
+  constexpr int NR = TY * LX + 10 * TX;  // cross-shaped staging region
+  constexpr int NS = (NR + NT - 1) / NT;
   constexpr int NQ = MHD ? 6 : 2;
-  __shared__ T L[3][LY][LX];
+  constexpr int NV = MHD ? 8 : 5;
+  // LDS planes: 0 vx (x), 1 vy (y), 2 p (xy); MHD: 3 F3 (xy), 4 F2 (x),
+  // 5 F1 (y), 6 by (x), 7 bz (xy), 8 bx (y)
+  constexpr int NL = MHD ? 9 : 3;
+  __shared__ T L[NL][LY * LX];
   __shared__ double rs[3][NT / 64 + 1];
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
-  const long fs = A.fs;
+  const uint32_t fs = (uint32_t)A.fs;
+  const uint32_t nxy = (uint32_t)nx * (uint32_t)ny;
   const bool first = A.first, last = A.last, periodic = A.periodic;
-  const T beta = A.beta;
 
-  // XCD-aware block order: hardware hands consecutive block ids to the 8
-  // XCDs in turn; remap so each XCD gets a contiguous run of tiles (shared
-  // halos stay in that XCD's L2).
+  // XCD-aware block order: the hardware hands consecutive block ids to the
+  // 8 XCDs in turn; remap so each XCD gets a contiguous run of tiles and the
+  // halos the tiles share stay in that XCD's L2.
   const int nblk = gridDim.x, b = blockIdx.x;
   const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
   const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
@@ -48,134 +62,152 @@ __global__ __launch_bounds__(TX* TY) void k_stage(const StageArgs<T> A) {
   const int k1 = min(k0 + A.kc, nz);
   const int i = i0 + tx, j = j0 + ty;
   const bool active = (i < nx) && (j < ny);
+  const uint32_t ij = active ? (uint32_t)(i + nx * j) : 0u;
 
-  const T* __restrict__ a = A.a;
-  const T* __restrict__ kin = A.kin;
-
-  auto U = [&](int f, long g) -> T {
-    T v = a[f * fs + g];
-    if (!first) v = v + beta * kin[f * fs + g];
-    return v;
-  };
-  auto val = [&](int kind, long g) -> T {
-    if constexpr (MHD) {
-      if (kind == K_F1) return -U(VZ, g) * A.boy[g] + U(VY, g) * A.boz[g];
-      if (kind == K_F2) return -U(VX, g) * A.boz[g] + U(VZ, g) * A.box[g];
-      if (kind == K_F3) return -U(VY, g) * A.box[g] + U(VX, g) * A.boy[g];
+  // ---- staging plan: per slot the LDS cell, the global xy offset and
+  // which halo (x or y) the cell belongs to ----
+  uint32_t s_off[NS];
+  int s_lds[NS], s_kind[NS];  // kind: 0 none, 1 centre, 2 x halo, 3 y halo
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int r = tid + s * NT;
+    int row, col;
+    if (r < TY * LX) {
+      row = 5 + r / LX;
+      col = r % LX;
+    } else {
+      const int r2 = r - TY * LX;
+      const int hr = r2 / TX;
+      row = hr < 5 ? hr : hr + TY;
+      col = 5 + r2 % TX;
     }
-    return U(kind, g);
-  };
-  // Global index of (gi, gj, k) with periodic wrap; -1 if outside a
-  // non-periodic domain (such cells are never read by a derivative).
-  auto gidx = [&](int gi, int gj, int k) -> long {
+    int gi = i0 - 5 + col, gj = j0 - 5 + row;
+    bool ok = r < NR;
     if (periodic) {
       gi = wrap(gi, nx);
       gj = wrap(gj, ny);
-    } else if (gi < 0 || gi >= nx || gj < 0 || gj >= ny) {
-      return -1;
+    } else {
+      ok = ok && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
     }
-    return (long)gi + (long)nx * ((long)gj + (long)ny * k);
-  };
-  auto ld = [&](int kind, int gi, int gj, int k) -> T {
-    long g = gidx(gi, gj, k);
-    return g < 0 ? T(0) : val(kind, g);
-  };
-  // LDS staging of one plane of one field: X = x halo, Y = y halo, XY = both.
-  auto stage_x = [&](int bf, int kind, int k) {
-    for (int l = tid; l < TY * LX; l += NT) {
-      int r = l / LX, c = l % LX;
-      L[bf][r + 5][c] = ld(kind, i0 - 5 + c, j0 + r, k);
-    }
-  };
-  auto stage_y = [&](int bf, int kind, int k) {
-    for (int l = tid; l < LY * TX; l += NT) {
-      int r = l / TX, c = l % TX;
-      L[bf][r][c + 5] = ld(kind, i0 + c, j0 - 5 + r, k);
-    }
-  };
-  auto stage_xy = [&](int bf, int kind, int k) {
-    stage_x(bf, kind, k);
-    for (int l = tid; l < 10 * TX; l += NT) {
-      int r = l / TX, c = l % TX;
-      int rr = r < 5 ? r : r + TY;
-      L[bf][rr][c + 5] = ld(kind, i0 + c, j0 - 5 + rr, k);
+    const bool xh = col < 5 || col >= TX + 5, yh = row < 5 || row >= TY + 5;
+    s_kind[s] = !ok ? 0 : (xh ? 2 : (yh ? 3 : 1));
+    s_off[s] = ok ? (uint32_t)(gi + nx * gj) : 0u;
+    s_lds[s] = row * LX + col;
+  }
+
+  // Buffer views.  The lambdas below capture only locals: capturing the
+  // kernel argument struct would force a private copy of it.
+  const T* const u = A.u;
+  const Buf<T> box(A.box, fs), boy(A.boy, fs), boz(A.boz, fs);
+  const T sxs = A.sxs, sys = A.sys;
+  auto UF = [&](int f, uint32_t g) -> T { return Buf<T>(u + (size_t)f * fs, fs)[g]; };
+  auto flux1 = [&](uint32_t g) -> T { return -UF(VZ, g) * boy[g] + UF(VY, g) * boz[g]; };
+  auto flux2 = [&](uint32_t g) -> T { return -UF(VX, g) * boz[g] + UF(VZ, g) * box[g]; };
+
+  auto stage_plane = [&](int k) {
+    const uint32_t pk = (uint32_t)k * nxy;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int kind = s_kind[s];
+      if (kind == 0) continue;
+      const uint32_t g = pk + s_off[s];
+      const int c = s_lds[s];
+      const bool wx = kind != 3, wy = kind != 2;  // feeds x / y derivatives
+      const T vx = UF(VX, g), vy = UF(VY, g), p = UF(P, g);
+      if (wx) L[0][c] = vx;
+      if (wy) L[1][c] = vy;
+      L[2][c] = p;
+      if constexpr (MHD) {
+        const T vz = UF(VZ, g), bx = UF(BX, g), by = UF(BY, g), bz = UF(BZ, g);
+        const T b0x = box[g], b0y = boy[g], b0z = boz[g];
+        L[3][c] = -vy * b0x + vx * b0y;           // F3 = -vy box + vx boy
+        if (wx) L[4][c] = -vx * b0z + vz * b0x;   // F2
+        if (wy) L[5][c] = -vz * b0y + vy * b0z;   // F1
+        if (wx) L[6][c] = by;
+        L[7][c] = bz;
+        if (wy) L[8][c] = bx;
+      }
     }
   };
   // 11-point derivative along x / y from LDS (derivatives.f90:38-44,66-75,92-101)
-  auto dx_lds = [&](int bf) -> T {
-    const T* row = &L[bf][ty + 5][0];
+  auto dx_lds = [&](int f) -> T {
+    const T* row = &L[f][(ty + 5) * LX];
     T acc = 0;
     if (!periodic && i < 5) {
 #pragma unroll
       for (int m = 0; m < 11; ++m) acc += row[min(m - i0 + 5, LX - 1)] * Coef<T>::b(i, m);
-      return A.sxs * acc;
+      return sxs * acc;
     }
     if (!periodic && i >= nx - 5) {
       const int r = nx - 1 - i;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) {
-        int c = min(max(nx - 1 - m - i0 + 5, 0), LX - 1);
-        acc += row[c] * Coef<T>::b(r, m);
-      }
-      return -A.sxs * acc;
+      for (int m = 0; m < 11; ++m) acc += row[min(max(nx - 1 - m - i0 + 5, 0), LX - 1)] * Coef<T>::b(r, m);
+      return -sxs * acc;
     }
     const int c0 = tx + 5;
 #pragma unroll
     for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (row[c0 + m] - row[c0 - m]);
-    return A.sxs * acc;
+    return sxs * acc;
   };
-  auto dy_lds = [&](int bf) -> T {
-    const int c0 = tx + 5;
+  auto dy_lds = [&](int f) -> T {
+    const T* col = &L[f][tx + 5];
     T acc = 0;
     if (!periodic && j < 5) {
 #pragma unroll
-      for (int m = 0; m < 11; ++m) acc += L[bf][min(max(m - j0 + 5, 0), LY - 1)][c0] * Coef<T>::b(j, m);
-      return A.sys * acc;
+      for (int m = 0; m < 11; ++m) acc += col[min(max(m - j0 + 5, 0), LY - 1) * LX] * Coef<T>::b(j, m);
+      return sys * acc;
     }
     if (!periodic && j >= ny - 5) {
       const int r = ny - 1 - j;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) {
-        int rr = min(max(ny - 1 - m - j0 + 5, 0), LY - 1);
-        acc += L[bf][rr][c0] * Coef<T>::b(r, m);
-      }
-      return -A.sys * acc;
+      for (int m = 0; m < 11; ++m) acc += col[min(max(ny - 1 - m - j0 + 5, 0), LY - 1) * LX] * Coef<T>::b(r, m);
+      return -sys * acc;
     }
     const int r0 = ty + 5;
 #pragma unroll
-    for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (L[bf][r0 + m][c0] - L[bf][r0 - m][c0]);
-    return A.sys * acc;
+    for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (col[(r0 + m) * LX] - col[(r0 - m) * LX]);
+    return sys * acc;
   };
 
-  // z register queue: slot m holds plane k-5+m of each z-differentiated field.
-  // Quiet: vz, p.  MHD: vz, p, bx, by, flux1, flux2.
+  // z register queue: slot m holds plane k-5+m of each z-differentiated
+  // field.  Quiet: vz, p.  MHD: vz, p, bx, by, flux1, flux2.
   T q[NQ][11];
-  auto load_plane = [&](int kk, int slot) {
+  T nq[NQ];  // next plane, loaded one iteration ahead
+  auto fetch = [&](int kk, T* dst) {
     if (active && kk >= 0 && kk < nz) {
-      const long g = (long)i + (long)nx * ((long)j + (long)ny * kk);
-      const T vz = U(VZ, g), p = U(P, g);
-      q[0][slot] = vz;
-      q[1][slot] = p;
+      const uint32_t g = ij + (uint32_t)kk * nxy;
+      dst[0] = UF(VZ, g);
+      dst[1] = UF(P, g);
       if constexpr (MHD) {
-        const T vx = U(VX, g), vy = U(VY, g);
-        q[2][slot] = U(BX, g);
-        q[3][slot] = U(BY, g);
-        q[4][slot] = -vz * A.boy[g] + vy * A.boz[g];
-        q[5][slot] = -vx * A.boz[g] + vz * A.box[g];
+        dst[2] = UF(BX, g);
+        dst[3] = UF(BY, g);
+        const T vx = UF(VX, g), vy = UF(VY, g), vz = dst[0];
+        dst[4] = -vz * boy[g] + vy * boz[g];
+        dst[5] = -vx * boz[g] + vz * box[g];
       }
     } else {
 #pragma unroll
-      for (int f = 0; f < NQ; ++f) q[f][slot] = 0;  // zero ghosts below the bottom
+      for (int f = 0; f < NQ; ++f) dst[f] = 0;  // zero ghosts below the bottom
     }
   };
 #pragma unroll
-  for (int m = 0; m < 10; ++m) load_plane(k0 - 5 + m, m);
+  for (int m = 0; m < 10; ++m) {
+    T t[NQ];
+    fetch(k0 - 5 + m, t);
+#pragma unroll
+    for (int f = 0; f < NQ; ++f) q[f][m] = t[f];
+  }
+  fetch(k0 + 5, nq);
 
-  constexpr int qkind[6] = {K_VZ, K_P, K_BX, K_BY, K_F1, K_F2};
   double ke = 0.0, mr = 1.0e300, mp = 1.0e300;
+  const T dt = A.dt, bnext = A.beta_next;
+  const uint32_t nbg = BG3D ? fs : (uint32_t)nz;
 
   for (int k = k0; k < k1; ++k) {
-    load_plane(k + 5, 10);
+    stage_plane(k);
+#pragma unroll
+    for (int f = 0; f < NQ; ++f) q[f][10] = nq[f];
+    fetch(k + 6, nq);
 
     // ---- z derivatives (derivatives.f90:481-487,509-518,536-545) ----
     T dz[NQ];
@@ -190,149 +222,136 @@ __global__ __launch_bounds__(TX* TY) void k_stage(const StageArgs<T> A) {
           dz[f] = st * acc;
         }
       } else {
+        // top block: mirrored one-sided rows over planes nz-1 .. nz-11
         const int r = nz - 1 - k;
+        T acc[NQ];
 #pragma unroll
-        for (int f = 0; f < NQ; ++f) {
-          T acc = 0;
-          for (int m = 0; m < 11; ++m) {
-            const long g = (long)i + (long)nx * ((long)j + (long)ny * (nz - 1 - m));
-            acc += val(qkind[f], g) * Coef<T>::b(r, m);
+        for (int f = 0; f < NQ; ++f) acc[f] = 0;
+        for (int m = 0; m < 11; ++m) {
+          const uint32_t g = ij + (uint32_t)(nz - 1 - m) * nxy;
+          const T c = Coef<T>::b(r, m);
+          acc[0] += UF(VZ, g) * c;
+          acc[1] += UF(P, g) * c;
+          if constexpr (MHD) {
+            acc[2] += UF(BX, g) * c;
+            acc[3] += UF(BY, g) * c;
+            acc[4] += flux1(g) * c;
+            acc[5] += flux2(g) * c;
           }
-          dz[f] = -st * acc;
         }
+#pragma unroll
+        for (int f = 0; f < NQ; ++f) dz[f] = -st * acc[f];
       }
-    }
-
-    // ---- x/y derivatives through LDS ----
-    T dvxdx, dvydy, gpx, gpy, vx, vy;
-    T dxf3, dyf3, dxf2, dyf1, dxby, dxbz, dybz, dybx, bz;
-    stage_x(0, K_VX, k);
-    stage_y(1, K_VY, k);
-    stage_xy(2, K_P, k);
-    __syncthreads();
-    if (active) {
-      dvxdx = dx_lds(0);
-      dvydy = dy_lds(1);
-      gpx = dx_lds(2);
-      gpy = dy_lds(2);
-      vx = L[0][ty + 5][tx + 5];
-      vy = L[1][ty + 5][tx + 5];
     }
     __syncthreads();
-    if constexpr (MHD) {
-      stage_xy(0, K_F3, k);
-      stage_x(1, K_F2, k);
-      stage_y(2, K_F1, k);
-      __syncthreads();
-      if (active) {
-        dxf3 = dx_lds(0);
-        dyf3 = dy_lds(0);
-        dxf2 = dx_lds(1);
-        dyf1 = dy_lds(2);
-      }
-      __syncthreads();
-      stage_x(0, K_BY, k);
-      stage_xy(1, K_BZ, k);
-      stage_y(2, K_BX, k);
-      __syncthreads();
-      if (active) {
-        dxby = dx_lds(0);
-        dxbz = dx_lds(1);
-        dybz = dy_lds(1);
-        dybx = dy_lds(2);
-        bz = L[1][ty + 5][tx + 5];
-      }
-      __syncthreads();
-    }
 
     if (active) {
-      const long g = (long)i + (long)nx * ((long)j + (long)ny * k);
-      const long gb = BG3D ? g : (long)k;
-      const T rho = U(RHO, g);
+      const uint32_t g = ij + (uint32_t)k * nxy;
+      const uint32_t gb = BG3D ? g : (uint32_t)k;
+      // xy derivatives and centre values from LDS
+      T dvxdx = dx_lds(0), dvydy = dy_lds(1), gpx = dx_lds(2), gpy = dy_lds(2);
+      const int cc = (ty + 5) * LX + tx + 5;
+      const T vx = L[0][cc], vy = L[1][cc];
+      const T rho = UF(RHO, g);
       const T vz = q[0][5], p = q[1][5];
       T dvzdz = dz[0], gpz = dz[1];
+      T dxf3 = 0, dyf3 = 0, dxf2 = 0, dyf1 = 0, dxby = 0, dxbz = 0, dybz = 0, dybx = 0, bz = 0;
       T dzbx = 0, dzby = 0, dzf1 = 0, dzf2 = 0;
       if constexpr (MHD) {
+        dxf3 = dx_lds(3);
+        dyf3 = dy_lds(3);
+        dxf2 = dx_lds(4);
+        dyf1 = dy_lds(5);
+        dxby = dx_lds(6);
+        dxbz = dx_lds(7);
+        dybz = dy_lds(7);
+        dybx = dy_lds(8);
+        bz = L[7][cc];
         dzbx = dz[2];
         dzby = dz[3];
         dzf1 = dz[4];
         dzf2 = dz[5];
       }
       // ---- CPML (RHS.f90:50-93, 424-546) ----
-      const T dt = A.dt;
+#pragma unroll
       for (int fz = ZBOT; fz <= ZTOP; ++fz) {
         const Face<T>& F = A.face[fz];
-        if (!F.active) continue;
         const int kp = k - F.off;
-        if (kp < 0 || kp >= F.n) continue;
-        const long fp = (long)i + (long)nx * ((long)j + (long)ny * kp);
-        pml_family(F, fp, 1, gpz, first, last, beta, dt);
-        pml_family(F, fp, 0, dvzdz, first, last, beta, dt);
+        if (!F.active || kp < 0 || kp >= F.n) continue;
+        const uint32_t fp = ij + nxy * (uint32_t)kp;
+        pml_family(F, fp, 1, gpz, first, last, bnext, dt);
+        pml_family(F, fp, 0, dvzdz, first, last, bnext, dt);
         if constexpr (MHD) {
-          pml_family(F, fp, 4, dzf2, first, last, beta, dt);
-          pml_family(F, fp, 5, dzf1, first, last, beta, dt);
-          pml_family(F, fp, 2, dzbx, first, last, beta, dt);
-          pml_family(F, fp, 3, dzby, first, last, beta, dt);
+          pml_family(F, fp, 4, dzf2, first, last, bnext, dt);
+          pml_family(F, fp, 5, dzf1, first, last, bnext, dt);
+          pml_family(F, fp, 2, dzbx, first, last, bnext, dt);
+          pml_family(F, fp, 3, dzby, first, last, bnext, dt);
         }
       }
+#pragma unroll
       for (int fx = XBOT; fx <= XTOP; ++fx) {
         const Face<T>& F = A.face[fx];
-        if (!F.active) continue;
         const int ip = i - F.off;
-        if (ip < 0 || ip >= F.n) continue;
-        const long fp = (long)ip + (long)F.n * ((long)j + (long)ny * k);
-        pml_family(F, fp, 1, gpx, first, last, beta, dt);
-        pml_family(F, fp, 0, dvxdx, first, last, beta, dt);
+        if (!F.active || ip < 0 || ip >= F.n) continue;
+        const uint32_t fp = (uint32_t)(ip + F.n * (j + ny * k));
+        pml_family(F, fp, 1, gpx, first, last, bnext, dt);
+        pml_family(F, fp, 0, dvxdx, first, last, bnext, dt);
         if constexpr (MHD) {
-          pml_family(F, fp, 4, dxf3, first, last, beta, dt);
-          pml_family(F, fp, 5, dxf2, first, last, beta, dt);
-          pml_family(F, fp, 2, dxby, first, last, beta, dt);
-          pml_family(F, fp, 3, dxbz, first, last, beta, dt);
+          pml_family(F, fp, 4, dxf3, first, last, bnext, dt);
+          pml_family(F, fp, 5, dxf2, first, last, bnext, dt);
+          pml_family(F, fp, 2, dxby, first, last, bnext, dt);
+          pml_family(F, fp, 3, dxbz, first, last, bnext, dt);
         }
       }
+#pragma unroll
       for (int fy = YBOT; fy <= YTOP; ++fy) {
         const Face<T>& F = A.face[fy];
-        if (!F.active) continue;
         const int jp = j - F.off;
-        if (jp < 0 || jp >= F.n) continue;
-        const long fp = (long)i + (long)nx * ((long)jp + (long)F.n * k);
-        pml_family(F, fp, 1, gpy, first, last, beta, dt);
-        pml_family(F, fp, 0, dvydy, first, last, beta, dt);
+        if (!F.active || jp < 0 || jp >= F.n) continue;
+        const uint32_t fp = (uint32_t)(i + nx * (jp + F.n * k));
+        pml_family(F, fp, 1, gpy, first, last, bnext, dt);
+        pml_family(F, fp, 0, dvydy, first, last, bnext, dt);
         if constexpr (MHD) {
-          pml_family(F, fp, 4, dyf3, first, last, beta, dt);
-          pml_family(F, fp, 5, dyf1, first, last, beta, dt);
-          pml_family(F, fp, 2, dybx, first, last, beta, dt);
-          pml_family(F, fp, 3, dybz, first, last, beta, dt);
+          pml_family(F, fp, 4, dyf3, first, last, bnext, dt);
+          pml_family(F, fp, 5, dyf1, first, last, bnext, dt);
+          pml_family(F, fp, 2, dybx, first, last, bnext, dt);
+          pml_family(F, fp, 3, dybz, first, last, bnext, dt);
         }
       }
 
       // ---- pointwise RHS ----
-      const T rho0 = A.rho0[gb], rhoinv = A.rhoinv[gb], sp = A.sponge[gb];
+      const Buf<T> brho0(A.rho0, nbg), brhoinv(A.rhoinv, nbg), bsp(A.sponge, nbg);
+      const T rho0 = brho0[gb], rhoinv = brhoinv[gb], sp = bsp[gb];
       const T div = dvxdx + dvydy + dvzdz;
       T K[8];
       if constexpr (!MHD) {
+        const Buf<T> bgr0z(A.gr0z, nbg), bc2r(A.c2rho0, nbg), bgp0z(A.gp0z, nbg);
         // RHS.f90:95-101,136,152
-        K[RHO] = -A.gr0z[gb] * vz - rho0 * div - sp * rho;
+        K[RHO] = -bgr0z[gb] * vz - rho0 * div - sp * rho;
         K[VX] = -rhoinv * gpx - sp * vx;
         K[VY] = -rhoinv * gpy - sp * vy;
         K[VZ] = -rhoinv * (gpz + rho * A.g[k]) - vz * sp;
-        K[P] = -A.c2rho0[gb] * div - vz * A.gp0z[gb] - sp * p;
+        K[P] = -bc2r[gb] * div - vz * bgp0z[gb] - sp * p;
       } else {
-        const T red = A.red[g];
-        const T b0x = A.box[g], b0y = A.boy[g], b0z = A.boz[g];
+        const Buf<T> bred(A.red, fs), bcbox(A.cbox, fs), bcboy(A.cboy, fs), bcboz(A.cboz, fs);
+        const Buf<T> bgr0x(A.gr0x, fs), bgr0y(A.gr0y, fs), bgr0z(A.gr0z, fs);
+        const Buf<T> bgp0x(A.gp0x, fs), bgp0y(A.gp0y, fs), bgp0z(A.gp0z, fs), bc2r(A.c2rho0, fs);
+        const T red = bred[g];
+        const T b0x = box[g], b0y = boy[g], b0z = boz[g];
         const T bx = q[2][5], by = q[3][5];
         // RHS.f90:548-580
         K[BX] = red * (dyf3 - dzf2);
         K[BY] = red * (dzf1 - dxf3);
         K[BZ] = red * (dxf2 - dyf1);
         const T cbx = dybz - dzby, cby = -dxbz + dzbx, cbz = dxby - dybx;
-        K[RHO] = -A.gr0x[g] * vx - A.gr0y[g] * vy - A.gr0z[g] * vz - rho0 * div - sp * rho;
-        K[P] = -A.c2rho0[g] * div - vx * A.gp0x[g] - vy * A.gp0y[g] - vz * A.gp0z[g] - sp * p;
+        K[RHO] = -bgr0x[g] * vx - bgr0y[g] * vy - bgr0z[g] * vz - rho0 * div - sp * rho;
+        K[P] = -bc2r[g] * div - vx * bgp0x[g] - vy * bgp0y[g] - vz * bgp0z[g] - sp * p;
         // CROSS(curl b, B0) and CROSS(curl B0, b), derivatives.f90:646-655
         const T c1 = cby * b0z - cbz * b0y, c2 = b0x * cbz - cbx * b0z, c3 = cbx * b0y - cby * b0x;
-        const T e1 = A.cboy[g] * bz - A.cboz[g] * by;
-        const T e2 = bx * A.cboz[g] - A.cbox[g] * bz;
-        const T e3 = A.cbox[g] * by - A.cboy[g] * bx;
+        const T cox = bcbox[g], coy = bcboy[g], coz = bcboz[g];
+        const T e1 = coy * bz - coz * by;
+        const T e2 = bx * coz - cox * bz;
+        const T e3 = cox * by - coy * bx;
         K[VX] = -rhoinv * (gpx - c1 * red) - sp * vx + rhoinv * e1 * red;
         K[VY] = -rhoinv * (gpy - c2 * red) - sp * vy + rhoinv * e2 * red;
         K[VZ] = -rhoinv * (gpz - c3 * red) - sp * vz + rhoinv * e3 * red - rhoinv * rho * A.g[k];
@@ -345,32 +364,39 @@ __global__ __launch_bounds__(TX* TY) void k_stage(const StageArgs<T> A) {
         else K[VZ] += s;
       }
       // boundary-plane zeroing (RHS.f90:154-164,647-657)
-      constexpr int NV = MHD ? 8 : 5;
       const bool zero = (k == 0) || (k == nz - 1) ||
                         (!periodic && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1));
       if (zero) {
 #pragma unroll
         for (int f = 0; f < NV; ++f) K[f] = 0;
       }
+      T* const ap = A.a;
       if (!last) {
+        // next stage value, step.f90:33
+        T* const un = A.un;
 #pragma unroll
-        for (int f = 0; f < NV; ++f) A.kout[f * fs + g] = K[f];
+        for (int f = 0; f < NV; ++f) {
+          const Buf<T> af(ap + (size_t)f * fs, fs), uf(un + (size_t)f * fs, fs);
+          uf.st(g, af[g] + bnext * K[f]);
+        }
       } else {
         T an[NV];
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
-          const T a0 = a[f * fs + g];
+          const Buf<T> af(ap + (size_t)f * fs, fs);
+          const T a0 = af[g];
           if constexpr (sizeof(T) == 4) {
-            // compensated update: the running sum a - c carries the lost bits
-            const T c0 = A.comp[f * fs + g];
-            const T y = A.dt * K[f] - c0;
+            // compensated update: a - c carries the bits a + dt K loses
+            const Buf<T> cf(A.comp + (size_t)f * fs, fs);
+            const T c0 = cf[g];
+            const T y = dt * K[f] - c0;
             const T t = a0 + y;
-            A.comp[f * fs + g] = (t - a0) - y;
+            cf.st(g, (t - a0) - y);
             an[f] = t;
           } else {
-            an[f] = a0 + A.dt * K[f];
+            an[f] = a0 + dt * K[f];
           }
-          A.aout[f * fs + g] = an[f];
+          af.st(g, an[f]);
         }
         // diagnostics: KE norm on the new state, minima on the stage-5 state
         const double m = ((double)rho0 + (double)an[RHO]) *
@@ -378,9 +404,10 @@ __global__ __launch_bounds__(TX* TY) void k_stage(const StageArgs<T> A) {
                          2.0;
         ke += m * m;
         mr = fmin(mr, (double)rho + (double)rho0);
-        mp = fmin(mp, (double)p + (double)A.p0[gb]);
+        mp = fmin(mp, (double)p + (double)Buf<T>(A.p0, nbg)[gb]);
       }
     }
+    __syncthreads();
 
 #pragma unroll
     for (int f = 0; f < NQ; ++f)
