@@ -1,0 +1,220 @@
+! sparc_hip.f90 -- ISO_C_BINDING interface to libsparc_hip.so for the SPARC2
+! Fortran host.  Types mirror the C structs of include/sparc_hip.h member by
+! member (bind(C) gives them the C layout); every function returns an
+! integer(c_int) status, 0 on success.
+!
+! The host keeps its own READ_SPARC_PARAMETERS, READ_IN_HDF5 and
+! INITIALISE_RHS (initialize.f90, all_modules.f90) and hands their results to
+! the context once; STEP (step.f90:20-63) and FILTER_VARS
+! (derivatives.f90:659-703) then become sparc_hip_step / sparc_hip_filter.
+module sparc_hip
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: SPARC_OK = 0, SPARC_E_INVALID = -1, &
+       SPARC_E_HIP = -2, SPARC_E_NOMEM = -3, SPARC_E_STATE = -4
+  integer(c_int), parameter, public :: SPARC_PREC_FP64 = 0, SPARC_PREC_FP32 = 1
+  integer(c_int), parameter, public :: SPARC_XBOT = 0, SPARC_XTOP = 1, SPARC_YBOT = 2, &
+       SPARC_YTOP = 3, SPARC_ZBOT = 4, SPARC_ZTOP = 5
+
+  type, bind(C), public :: sparc_cfg
+    integer(c_int) :: nx, ny, nz
+    integer(c_int) :: option          ! 1 MP_QUIET_3D, 3 MP_MHD_3D
+    integer(c_int) :: precision
+    integer(c_int) :: periodic
+    integer(c_int) :: use_pml, use_hpml
+    integer(c_int) :: npml(6)         ! nxpmlbot, nxpmltop, nypmlbot, nypmltop, nzpmlbot, nzpmltop
+    integer(c_int) :: g_xy_filt, g_z_filt
+    real(c_double) :: s_xy, s_z
+    integer(c_int) :: xyfreq, zfreq
+    integer(c_int) :: pulse_dir
+    integer(c_int) :: bg3d
+    real(c_double) :: deltat
+    real(c_double) :: betas(5)
+    real(c_double) :: stretchx, stretchy
+    type(c_ptr) :: stretch            ! c_loc(stretch(1)), nz values
+    integer(c_int) :: device
+    integer(c_int) :: tile(3)
+  end type sparc_cfg
+
+  type, bind(C), public :: sparc_bg
+    type(c_ptr) :: rho0, rhoinv, c2rho0, gradrho0_z, gradp0_z, spongexyz, p0
+    type(c_ptr) :: g
+    type(c_ptr) :: box, boy, boz, curlbox, curlboy, curlboz
+    type(c_ptr) :: gradrho0_x, gradrho0_y, gradp0_x, gradp0_y, reduction
+    type(c_ptr) :: pml_a(6), pml_b(6), pml_kappa(6)
+    type(c_ptr) :: src_x, src_y, src_z
+    real(c_double) :: dimc, dimrho
+  end type sparc_bg
+
+  public :: sparc_hip_create, sparc_hip_destroy, sparc_hip_set_background, &
+       sparc_hip_put_state, sparc_hip_get_state, sparc_hip_step, sparc_hip_filter, &
+       sparc_hip_advance, sparc_hip_diag, sparc_hip_ddx, sparc_hip_ddy, sparc_hip_ddz, &
+       sparc_hip_filtx, sparc_hip_filty, sparc_hip_filtz, sparc_hip_sync, &
+       sparc_hip_timing, sparc_hip_kernel_time, sparc_hip_tile, sparc_hip_version, &
+       sparc_error_message
+
+  interface
+    integer(c_int) function sparc_hip_create(cfg, ctx) bind(C, name="sparc_hip_create")
+      import :: c_int, c_ptr, sparc_cfg
+      type(sparc_cfg), intent(in) :: cfg
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function sparc_hip_destroy(ctx) bind(C, name="sparc_hip_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function sparc_hip_set_background(ctx, bg) bind(C, name="sparc_hip_set_background")
+      import :: c_int, c_ptr, sparc_bg
+      type(c_ptr), value :: ctx
+      type(sparc_bg), intent(in) :: bg
+    end function
+
+    ! a: the reference's a(dim1,dim2,dim3,dimfive); psi(f): c_loc of the PML
+    ! bases (zpmlvars & co.) or c_null_ptr
+    integer(c_int) function sparc_hip_put_state(ctx, a, psi) bind(C, name="sparc_hip_put_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: a(*)
+      type(c_ptr), intent(in) :: psi(6)
+    end function
+
+    integer(c_int) function sparc_hip_get_state(ctx, a, psi) bind(C, name="sparc_hip_get_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: a(*)
+      type(c_ptr), intent(in) :: psi(6)
+    end function
+
+    integer(c_int) function sparc_hip_step(ctx, src_amp) bind(C, name="sparc_hip_step")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: src_amp
+    end function
+
+    integer(c_int) function sparc_hip_filter(ctx, time) bind(C, name="sparc_hip_filter")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: time
+    end function
+
+    integer(c_int) function sparc_hip_advance(ctx, n0, n1, src_amp) bind(C, name="sparc_hip_advance")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n0, n1
+      real(c_double), intent(in) :: src_amp(*)
+    end function
+
+    integer(c_int) function sparc_hip_diag(ctx, out) bind(C, name="sparc_hip_diag")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(3)
+    end function
+
+    integer(c_int) function sparc_hip_ddx(ctx, input, output, bc) bind(C, name="sparc_hip_ddx")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: input(*)
+      real(c_double), intent(out) :: output(*)
+      integer(c_int), value :: bc
+    end function
+
+    integer(c_int) function sparc_hip_ddy(ctx, input, output, bc) bind(C, name="sparc_hip_ddy")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: input(*)
+      real(c_double), intent(out) :: output(*)
+      integer(c_int), value :: bc
+    end function
+
+    integer(c_int) function sparc_hip_ddz(ctx, input, output, bc) bind(C, name="sparc_hip_ddz")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: input(*)
+      real(c_double), intent(out) :: output(*)
+      integer(c_int), value :: bc
+    end function
+
+    integer(c_int) function sparc_hip_filtx(ctx, input, output) bind(C, name="sparc_hip_filtx")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: input(*)
+      real(c_double), intent(out) :: output(*)
+    end function
+
+    integer(c_int) function sparc_hip_filty(ctx, input, output) bind(C, name="sparc_hip_filty")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: input(*)
+      real(c_double), intent(out) :: output(*)
+    end function
+
+    integer(c_int) function sparc_hip_filtz(ctx, input, output) bind(C, name="sparc_hip_filtz")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: input(*)
+      real(c_double), intent(out) :: output(*)
+    end function
+
+    integer(c_int) function sparc_hip_sync(ctx) bind(C, name="sparc_hip_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function sparc_hip_timing(ctx, enable) bind(C, name="sparc_hip_timing")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+
+    integer(c_int) function sparc_hip_kernel_time(ctx, kind, count, total_ms) &
+         bind(C, name="sparc_hip_kernel_time")
+      import :: c_int, c_ptr, c_long, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind
+      integer(c_long), intent(out) :: count
+      real(c_double), intent(out) :: total_ms
+    end function
+
+    integer(c_int) function sparc_hip_tile(ctx, tile, nblocks) bind(C, name="sparc_hip_tile")
+      import :: c_int, c_ptr, c_long
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: tile(3)
+      integer(c_long), intent(out) :: nblocks
+    end function
+
+    integer(c_int) function sparc_hip_version() bind(C, name="sparc_hip_version")
+      import :: c_int
+    end function
+
+    type(c_ptr) function sparc_hip_last_error_c() bind(C, name="sparc_hip_last_error")
+      import :: c_ptr
+    end function
+
+    integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+    end function
+  end interface
+
+contains
+
+  ! Message of the most recent failure, as a Fortran string.
+  function sparc_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, i
+    p = sparc_hip_last_error_c()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = chars(i)
+    end do
+  end function sparc_error_message
+
+end module sparc_hip
