@@ -139,6 +139,21 @@ int sparc_hip_kernel_time(sparc_ctx* ctx, int kind, long* count, double* total_m
 /* Tile (x, y, z-chunk) the stage kernel uses and its launch size. */
 int sparc_hip_tile(sparc_ctx* ctx, int tile[3], long* nblocks);
 
+/* Batched form: advance many independent runs (contexts on one device, one
+ * precision) together.  Each RK stage is one persistent launch per kernel
+ * variant and tile shape over the work items of all runs; with use_graph the
+ * launch sequence of one step is captured once in a HIP graph and replayed.
+ * No counterpart in the reference (one run per MPI job there); it serves
+ * callers that run many forward simulations at once.
+ * batch_advance: nsteps driver steps; time0[r] is run r's time index of the
+ * first of them, amps[r*nsteps + n] its source amplitude at step n.  Filters
+ * follow each run's own cadence.  Synchronises before returning. */
+typedef struct sparc_batch sparc_batch;
+int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, sparc_batch** out);
+int sparc_hip_batch_advance(sparc_batch* b, int nsteps, const int* time0, const double* amps);
+int sparc_hip_batch_info(sparc_batch* b, int* ngroups, long* nlaunch_blocks);
+int sparc_hip_batch_destroy(sparc_batch* b);
+
 const char* sparc_hip_strerror(int code);
 const char* sparc_hip_last_error(void);
 int sparc_hip_version(void);

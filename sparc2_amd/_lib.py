@@ -55,7 +55,8 @@ EXPORTS = [
     "sparc_hip_diag", "sparc_hip_ddx", "sparc_hip_ddy", "sparc_hip_ddz", "sparc_hip_filtx",
     "sparc_hip_filty", "sparc_hip_filtz", "sparc_hip_sync", "sparc_hip_timing",
     "sparc_hip_kernel_time", "sparc_hip_tile", "sparc_hip_strerror", "sparc_hip_last_error",
-    "sparc_hip_version",
+    "sparc_hip_version", "sparc_hip_batch_create", "sparc_hip_batch_advance",
+    "sparc_hip_batch_info", "sparc_hip_batch_destroy",
 ]
 
 _lib = None
@@ -93,6 +94,10 @@ def lib() -> C.CDLL:
     L.sparc_hip_timing.argtypes = [vp, i]
     L.sparc_hip_kernel_time.argtypes = [vp, i, C.POINTER(C.c_long), C.POINTER(C.c_double)]
     L.sparc_hip_tile.argtypes = [vp, C.c_int * 3, C.POINTER(C.c_long)]
+    L.sparc_hip_batch_create.argtypes = [C.POINTER(vp), i, i, C.POINTER(vp)]
+    L.sparc_hip_batch_advance.argtypes = [vp, i, C.POINTER(C.c_int), DP]
+    L.sparc_hip_batch_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_long)]
+    L.sparc_hip_batch_destroy.argtypes = [vp]
     L.sparc_hip_strerror.restype = C.c_char_p
     L.sparc_hip_strerror.argtypes = [i]
     L.sparc_hip_last_error.restype = C.c_char_p

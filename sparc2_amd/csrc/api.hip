@@ -28,6 +28,13 @@ hipError_t launch_deriv(int axis, const double* in, double* out, int nx, int ny,
                         const double* stretch, int bc, hipStream_t st);
 hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, hipStream_t st);
 hipError_t launch_zero(void* p, long n, bool f32, hipStream_t st);
+template <typename T>
+hipError_t launch_stage_batch(bool mhd, bool bg3d, int tx, int ty, const StageArgs<T>* runs,
+                              const int* item_start, int nrun, int nblk, hipStream_t s);
+int stage_batch_blocks(bool mhd, int tx, int ty, bool f32, int ncu);
+hipError_t launch_diag_reduce_batch(const void* jobs, int njobs, hipStream_t st);
+hipError_t launch_tick(int* c, int reset, hipStream_t st);
+hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st);
 }  // namespace sparc
 
 using namespace sparc;
@@ -460,9 +467,10 @@ static void timed_end(sparc_ctx* c, TimeRec* r) {
   c->trecs.push_back(*r);
 }
 
+// Stage arguments common to all five stages of one run.
 template <typename T>
-static int do_step(sparc_ctx* c, double amp) {
-  StageArgs<T> A{};
+static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
+  A = StageArgs<T>{};
   A.nx = c->nx;
   A.ny = c->ny;
   A.nz = c->nz;
@@ -477,7 +485,6 @@ static int do_step(sparc_ctx* c, double amp) {
   A.dt = (T)c->cfg.deltat;
   A.sxs = (T)c->cfg.stretchx;
   A.sys = (T)c->cfg.stretchy;
-  A.src_amp = (T)amp;
   auto B = [&](int s) { return (const T*)c->bg[s]; };
   A.rho0 = B(B_RHO0);
   A.rhoinv = B(B_RHOINV);
@@ -519,22 +526,34 @@ static int do_step(sparc_ctx* c, double amp) {
   }
   A.partial = c->partial;
   A.comp = c->f32 ? (T*)c->comp[c->rc] : nullptr;
-  // u_1 = a; stage s reads u_s and writes u_{s+1}; the last stage updates a
-  int uin = c->ra, uout = c->rk0;
   A.a = (T*)c->pool[c->ra];
+}
+
+// Stage s of LDDRK-5: u_1 = a; stage s reads u_s and writes
+// u_{s+1} = a + beta_{s+1} K_s into the other stage buffer; stage 5 updates a.
+template <typename T>
+static void set_stage(sparc_ctx* c, StageArgs<T>& A, int s) {
+  const int uin = s == 1 ? c->ra : (s % 2 == 0 ? c->rk0 : c->rk1);
+  const int uout = s % 2 == 1 ? c->rk0 : c->rk1;
+  A.first = s == 1;
+  A.last = s == 5;
+  A.beta_next = A.last ? (T)0 : (T)c->cfg.betas[s];
+  A.u = (const T*)c->pool[uin];
+  A.un = A.last ? nullptr : (T*)c->pool[uout];
+}
+
+template <typename T>
+static int do_step(sparc_ctx* c, double amp) {
+  StageArgs<T> A;
+  fill_common<T>(c, A);
+  A.src_amp = (T)amp;
   for (int s = 1; s <= 5; ++s) {
-    A.first = s == 1;
-    A.last = s == 5;
-    A.beta_next = A.last ? (T)0 : (T)c->cfg.betas[s];
-    A.u = (const T*)c->pool[uin];
-    A.un = A.last ? nullptr : (T*)c->pool[uout];
+    set_stage<T>(c, A, s);
     TimeRec r;
     timed_begin(c, A.first ? 0 : (A.last ? 2 : 1), &r);
     hipError_t e = launch_stage<T>(c->mhd, c->bg3d, c->tx, c->ty, A, c->st);
     timed_end(c, &r);
     if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("stage launch: ") + hipGetErrorString(e));
-    uin = uout;
-    uout = (uout == c->rk0) ? c->rk1 : c->rk0;
   }
   const int nblk = c->ntx * c->nty * c->nchunk;
   if (launch_diag_reduce(c->partial, nblk, c->diag, c->st) != hipSuccess)
@@ -702,3 +721,269 @@ int sparc_hip_tile(sparc_ctx* c, int tile[3], long* nblocks) {
   return SPARC_OK;
 }
 
+
+// ===========================================================================
+// Batched form: many independent runs (contexts) advanced together.  Runs are
+// grouped by kernel variant and tile shape; each RK stage is one persistent
+// launch per group over the work items of all its runs.  Stage arguments live
+// in device memory, the source amplitude comes from a per-run device table
+// indexed by a device step counter, so one step is the same launch sequence
+// every time and can be captured in a HIP graph.
+// ===========================================================================
+namespace {
+struct BatchGroup {
+  bool mhd = false, bg3d = false;
+  int tx = 0, ty = 0, nblk = 0;
+  std::vector<int> members;
+  void* args[5] = {};        // StageArgs<T>[members] per stage (device)
+  int* item_start = nullptr;  // prefix of work items (device)
+};
+}  // namespace
+
+struct sparc_batch {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  bool f32 = false, use_graph = false;
+  std::vector<sparc_ctx*> runs;
+  std::vector<BatchGroup> groups;
+  int* tstep = nullptr;
+  std::vector<void*> amp;  // per run, amp_cap entries (device)
+  int amp_cap = 0;
+  void* jobs = nullptr;    // DiagJob[nrun] (device)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+static void batch_free(sparc_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto& g : b->groups) {
+    for (auto& p : g.args)
+      if (p) (void)hipFree(p);
+    if (g.item_start) (void)hipFree(g.item_start);
+  }
+  for (auto& p : b->amp)
+    if (p) (void)hipFree(p);
+  if (b->tstep) (void)hipFree(b->tstep);
+  if (b->jobs) (void)hipFree(b->jobs);
+  if (b->st) (void)hipStreamDestroy(b->st);
+}
+
+template <typename T>
+static int batch_upload_args(sparc_batch* b) {
+  for (auto& g : b->groups) {
+    for (int s = 1; s <= 5; ++s) {
+      std::vector<StageArgs<T>> h(g.members.size());
+      for (size_t q = 0; q < g.members.size(); ++q) {
+        const int r = g.members[q];
+        sparc_ctx* c = b->runs[r];
+        fill_common<T>(c, h[q]);
+        set_stage<T>(c, h[q], s);
+        h[q].amp_tab = c->cfg.pulse_dir ? (const T*)b->amp[r] : nullptr;
+        h[q].tstep = b->tstep;
+      }
+      HIPCHK(hipMemcpy(g.args[s - 1], h.data(), sizeof(StageArgs<T>) * h.size(), hipMemcpyHostToDevice));
+    }
+  }
+  return SPARC_OK;
+}
+
+template <typename T>
+static int batch_enqueue_step(sparc_batch* b) {
+  for (int s = 1; s <= 5; ++s)
+    for (auto& g : b->groups) {
+      hipError_t e = launch_stage_batch<T>(g.mhd, g.bg3d, g.tx, g.ty, (const StageArgs<T>*)g.args[s - 1],
+                                           g.item_start, (int)g.members.size(), g.nblk, b->st);
+      if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch stage launch: ") + hipGetErrorString(e));
+    }
+  if (launch_diag_reduce_batch(b->jobs, (int)b->runs.size(), b->st) != hipSuccess ||
+      launch_tick(b->tstep, 0, b->st) != hipSuccess)
+    return fail(SPARC_E_HIP, "batch diag/tick launch failed");
+  return SPARC_OK;
+}
+
+// FILTER_VARS for one run on the batch stream, written back into the run's
+// own buffers (the device stage arguments keep pointing at them).
+template <typename T>
+static int batch_filter(sparc_batch* b, sparc_ctx* c, int time) {
+  auto pass = [&](int axis, int g, T sv, const void* in, void* out, bool comp) -> hipError_t {
+    (void)comp;
+    return launch_filter<T>(axis, g, (const T*)in, (T*)out, c->nx, c->ny, c->nz, c->fs, c->nv,
+                            c->cfg.periodic, sv, 0, b->st);
+  };
+  void* a = c->pool[c->ra];
+  void* t = c->pool[c->rk0];
+  const long n = (long)c->nv * c->fs;
+  hipError_t e = hipSuccess;
+  if (time % c->cfg.xyfreq == 0) {
+    const T sv = (T)c->cfg.s_xy;
+    e = pass(1, c->cfg.g_xy_filt, sv, a, t, false);
+    if (e == hipSuccess) e = pass(0, c->cfg.g_xy_filt, sv, t, a, false);
+    if (e == hipSuccess && c->f32) {
+      e = pass(1, c->cfg.g_xy_filt, sv, c->comp[c->rc], c->comp[1 - c->rc], true);
+      if (e == hipSuccess) e = pass(0, c->cfg.g_xy_filt, sv, c->comp[1 - c->rc], c->comp[c->rc], true);
+    }
+  }
+  if (e == hipSuccess && time % c->cfg.zfreq == 0) {
+    const T sv = (T)c->cfg.s_z;
+    e = pass(2, c->cfg.g_z_filt, sv, a, t, false);
+    if (e == hipSuccess) e = launch_copy(t, a, n, c->f32, b->st);
+    if (e == hipSuccess && c->f32) {
+      e = pass(2, c->cfg.g_z_filt, sv, c->comp[c->rc], c->comp[1 - c->rc], true);
+      if (e == hipSuccess) e = launch_copy(c->comp[1 - c->rc], c->comp[c->rc], n, true, b->st);
+    }
+  }
+  if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch filter: ") + hipGetErrorString(e));
+  return SPARC_OK;
+}
+
+int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, sparc_batch** out) {
+  if (!runs || nrun < 1 || !out) return fail(SPARC_E_INVALID, "batch needs at least one run");
+  *out = nullptr;
+  for (int r = 0; r < nrun; ++r) {
+    if (!runs[r]) return fail(SPARC_E_INVALID, "null run in batch");
+    if (!runs[r]->bg_set || !runs[r]->state_set)
+      return fail(SPARC_E_STATE, "every run needs set_background and put_state first");
+    if (runs[r]->dev != runs[0]->dev || runs[r]->f32 != runs[0]->f32)
+      return fail(SPARC_E_INVALID, "batched runs must share device and precision");
+    for (int q = 0; q < r; ++q)
+      if (runs[q] == runs[r]) return fail(SPARC_E_INVALID, "a run appears twice in the batch");
+  }
+  sparc_batch* b = new sparc_batch();
+  b->dev = runs[0]->dev;
+  b->f32 = runs[0]->f32;
+  b->use_graph = use_graph != 0;
+  b->runs.assign(runs, runs + nrun);
+  auto bail = [&](int code) {
+    batch_free(b);
+    delete b;
+    return code;
+  };
+  if (hipSetDevice(b->dev) != hipSuccess) return bail(fail(SPARC_E_HIP, "hipSetDevice failed"));
+  for (auto* c : b->runs)
+    if (hipStreamSynchronize(c->st) != hipSuccess) return bail(fail(SPARC_E_HIP, "run stream sync failed"));
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(SPARC_E_HIP, "hipStreamCreate failed"));
+  int ncu = 256;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->dev);
+  for (int r = 0; r < nrun; ++r) {
+    sparc_ctx* c = b->runs[r];
+    BatchGroup* g = nullptr;
+    for (auto& h : b->groups)
+      if (h.mhd == c->mhd && h.bg3d == c->bg3d && h.tx == c->tx && h.ty == c->ty) g = &h;
+    if (!g) {
+      b->groups.emplace_back();
+      g = &b->groups.back();
+      g->mhd = c->mhd;
+      g->bg3d = c->bg3d;
+      g->tx = c->tx;
+      g->ty = c->ty;
+    }
+    g->members.push_back(r);
+  }
+  const size_t asz = b->f32 ? sizeof(StageArgs<float>) : sizeof(StageArgs<double>);
+  int rc;
+  for (auto& g : b->groups) {
+    std::vector<int> start(g.members.size() + 1, 0);
+    for (size_t q = 0; q < g.members.size(); ++q) {
+      const sparc_ctx* c = b->runs[g.members[q]];
+      start[q + 1] = start[q] + c->ntx * c->nty * c->nchunk;
+    }
+    for (int s = 0; s < 5; ++s)
+      if ((rc = dalloc(&g.args[s], asz * g.members.size()))) return bail(rc);
+    if ((rc = dalloc((void**)&g.item_start, sizeof(int) * start.size()))) return bail(rc);
+    if (hipMemcpy(g.item_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(SPARC_E_HIP, "item table upload failed"));
+    g.nblk = std::min(stage_batch_blocks(g.mhd, g.tx, g.ty, b->f32, ncu), (start.back() + 7) / 8 * 8);
+  }
+  if ((rc = dalloc((void**)&b->tstep, sizeof(int)))) return bail(rc);
+  std::vector<DiagJob> jobs(nrun);
+  for (int r = 0; r < nrun; ++r) {
+    sparc_ctx* c = b->runs[r];
+    jobs[r].partial = c->partial;
+    jobs[r].out = c->diag;
+    jobs[r].nblk = c->ntx * c->nty * c->nchunk;
+    jobs[r].pad_ = 0;
+  }
+  if ((rc = dalloc(&b->jobs, sizeof(DiagJob) * nrun))) return bail(rc);
+  if (hipMemcpy(b->jobs, jobs.data(), sizeof(DiagJob) * nrun, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(SPARC_E_HIP, "diag job upload failed"));
+  b->amp.assign(nrun, nullptr);
+  *out = b;
+  return SPARC_OK;
+}
+
+int sparc_hip_batch_destroy(sparc_batch* b) {
+  if (!b) return SPARC_OK;
+  (void)hipSetDevice(b->dev);
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  batch_free(b);
+  delete b;
+  return SPARC_OK;
+}
+
+template <typename T>
+static int batch_advance(sparc_batch* b, int nsteps, const int* time0, const double* amps) {
+  const int nrun = (int)b->runs.size();
+  int rc;
+  if (nsteps > b->amp_cap) {  // grow the amplitude tables (device pointers change)
+    for (auto& p : b->amp)
+      if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+      }
+    for (int r = 0; r < nrun; ++r)
+      if ((rc = dalloc(&b->amp[r], sizeof(T) * nsteps))) return rc;
+    b->amp_cap = nsteps;
+  }
+  for (int r = 0; r < nrun; ++r) {
+    if (!b->runs[r]->cfg.pulse_dir) continue;
+    std::vector<T> h(nsteps);
+    for (int n = 0; n < nsteps; ++n) h[n] = (T)amps[(long)r * nsteps + n];
+    HIPCHK(hipMemcpy(b->amp[r], h.data(), sizeof(T) * nsteps, hipMemcpyHostToDevice));
+  }
+  if ((rc = batch_upload_args<T>(b))) return rc;
+  if (launch_tick(b->tstep, 1, b->st) != hipSuccess) return fail(SPARC_E_HIP, "tick launch failed");
+  if (b->use_graph && !b->gexec) {
+    HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    rc = batch_enqueue_step<T>(b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->st, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  }
+  for (int n = 0; n < nsteps; ++n) {
+    if (b->gexec) {
+      HIPCHK(hipGraphLaunch(b->gexec, b->st));
+    } else if ((rc = batch_enqueue_step<T>(b))) {
+      return rc;
+    }
+    for (int r = 0; r < nrun; ++r) {
+      const int t = time0[r] + n;
+      sparc_ctx* c = b->runs[r];
+      if (t != 0 && (t % c->cfg.xyfreq == 0 || t % c->cfg.zfreq == 0))
+        if ((rc = batch_filter<T>(b, c, t))) return rc;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(b->st));
+  for (auto* c : b->runs) c->steps_done += nsteps;
+  return SPARC_OK;
+}
+
+int sparc_hip_batch_advance(sparc_batch* b, int nsteps, const int* time0, const double* amps) {
+  if (!b || nsteps < 0 || (nsteps > 0 && (!time0 || !amps))) return fail(SPARC_E_INVALID, "bad argument");
+  if (nsteps == 0) return SPARC_OK;
+  HIPCHK(hipSetDevice(b->dev));
+  return b->f32 ? batch_advance<float>(b, nsteps, time0, amps) : batch_advance<double>(b, nsteps, time0, amps);
+}
+
+int sparc_hip_batch_info(sparc_batch* b, int* ngroups, long* nlaunch_blocks) {
+  if (!b || !ngroups || !nlaunch_blocks) return fail(SPARC_E_INVALID, "null argument");
+  *ngroups = (int)b->groups.size();
+  long n = 0;
+  for (auto& g : b->groups) n += g.nblk;
+  *nlaunch_blocks = n;
+  return SPARC_OK;
+}

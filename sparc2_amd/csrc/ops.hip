@@ -173,6 +173,68 @@ __global__ __launch_bounds__(256) void k_diag_reduce(const double* __restrict__ 
   }
 }
 
+// Batched form: block r reduces run r's partials into its own out[3].
+
+__global__ __launch_bounds__(256) void k_diag_reduce_batch(const DiagJob* __restrict__ jobs) {
+  __shared__ double s0[256], s1[256], s2[256];
+  const DiagJob J = jobs[blockIdx.x];
+  const int t = threadIdx.x;
+  double a = 0.0, r = 1.0e300, p = 1.0e300;
+  for (int b = t; b < J.nblk; b += 256) {
+    a += J.partial[3 * (long)b];
+    r = fmin(r, J.partial[3 * (long)b + 1]);
+    p = fmin(p, J.partial[3 * (long)b + 2]);
+  }
+  s0[t] = a;
+  s1[t] = r;
+  s2[t] = p;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      s0[t] += s0[t + o];
+      s1[t] = fmin(s1[t], s1[t + o]);
+      s2[t] = fmin(s2[t], s2[t + o]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    J.out[0] = s0[0];
+    J.out[1] = s1[0];
+    J.out[2] = s2[0];
+  }
+}
+
+hipError_t launch_diag_reduce_batch(const void* jobs, int njobs, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_reduce_batch, dim3(njobs), dim3(256), 0, st, (const DiagJob*)jobs);
+  return hipGetLastError();
+}
+
+// Device step counter of the batched form: +1 after each step, 0 on reset.
+__global__ void k_tick(int* c, int reset) {
+  if (threadIdx.x == 0) c[0] = reset ? 0 : c[0] + 1;
+}
+
+hipError_t launch_tick(int* c, int reset, hipStream_t st) {
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, c, reset);
+  return hipGetLastError();
+}
+
+// Plain copy (the in-place filter of the batched form copies its result back).
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy(const T* __restrict__ in, T* __restrict__ out, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st) {
+  dim3 grid((unsigned)((n + 255) / 256));
+  if (f32)
+    hipLaunchKernelGGL(k_copy<float>, grid, dim3(256), 0, st, (const float*)in, (float*)out, n);
+  else
+    hipLaunchKernelGGL(k_copy<double>, grid, dim3(256), 0, st, (const double*)in, (double*)out, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_diag_reduce, dim3(1), dim3(256), 0, st, partial, nblk, out);
   return hipGetLastError();

@@ -104,6 +104,10 @@ template <typename T> struct StageArgs {
   int ntx, nty, nblk_xy, nchunk;
   int periodic, first, last, pulse_dir;
   T beta_next, dt, sxs, sys, src_amp;
+  // when amp_tab is set the source amplitude is amp_tab[*tstep] (a device
+  // table and step counter, so a captured step replays unchanged)
+  const T* amp_tab;
+  const int* tstep;
   const T* u;
   T* a;
   T* un;
@@ -115,6 +119,14 @@ template <typename T> struct StageArgs {
   const T *g, *stretch, *sx, *sy, *sz;
   Face<T> face[6];
   double* partial;  // last stage: 3 doubles per block (KE sum, min rho, min p)
+};
+
+// One run's diagnostic reduction in the batched form.
+struct DiagJob {
+  const double* partial;
+  double* out;
+  int nblk;
+  int pad_;
 };
 
 // PML update of one derivative family at one point (RHS.f90:52-55): the psi

@@ -21,16 +21,18 @@
 
 namespace sparc {
 
-// Occupancy target (waves per SIMD): the register queue of MHD in FP64
-// (6 fields x 11 planes) needs about 200 registers, so it runs 2 waves per
-// SIMD; the other variants are held to 3 so memory latency overlaps.
+// Occupancy target (waves per SIMD): the MHD register queue (6 fields x 11
+// planes) plus nine LDS-staged fields need about 250 registers, so MHD runs 2
+// waves per SIMD; the quiet variants are held to 3 so memory latency overlaps.
 template <typename T, bool MHD>
 constexpr int stage_waves() {
-  return MHD ? (sizeof(T) == 8 ? 2 : 3) : 3;
+  return MHD ? 2 : 3;
 }
 
+// One work item: tile `tile` of z chunk `chunk` (logical = chunk*nblk_xy + tile)
+// of the run described by A.
 template <typename T, bool MHD, bool BG3D, int TX, int TY>
-__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage(const StageArgs<T> A) {
+__device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logical) {
   constexpr int NT = TX * TY;
   constexpr int LX = TX + 10, LY = TY + 10;
   constexpr int NR = TY * LX + 10 * TX;  // cross-shaped staging region
@@ -50,12 +52,6 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage(const
   const uint32_t nxy = (uint32_t)nx * (uint32_t)ny;
   const bool first = A.first, last = A.last, periodic = A.periodic;
 
-  // XCD-aware block order: the hardware hands consecutive block ids to the
-  // 8 XCDs in turn; remap so each XCD gets a contiguous run of tiles and the
-  // halos the tiles share stay in that XCD's L2.
-  const int nblk = gridDim.x, b = blockIdx.x;
-  const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
-  const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
   const int tile = logical % A.nblk_xy, chunk = logical / A.nblk_xy;
   const int i0 = (tile % A.ntx) * TX, j0 = (tile / A.ntx) * TY;
   const int k0 = chunk * A.kc;
@@ -358,7 +354,8 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage(const
       }
       // pulse source (driver.f90:137-148 separable form; RHS.f90:138-145,632-639)
       if (A.pulse_dir >= 1 && A.pulse_dir <= 3) {
-        const T s = A.src_amp * A.sx[i] * A.sy[j] * A.sz[k];
+        const T amp = A.amp_tab ? A.amp_tab[*A.tstep] : A.src_amp;
+        const T s = amp * A.sx[i] * A.sy[j] * A.sz[k];
         if (A.pulse_dir == 1) K[VX] += s;
         else if (A.pulse_dir == 2) K[VY] += s;
         else K[VZ] += s;
@@ -444,6 +441,35 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage(const
   }
 }
 
+// Single run: one block per work item.  The hardware hands consecutive block
+// ids to the 8 XCDs in turn; remap so each XCD gets a contiguous run of tiles
+// and the halos the tiles share stay in that XCD's L2.
+template <typename T, bool MHD, bool BG3D, int TX, int TY>
+__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage(const StageArgs<T> A) {
+  const int nblk = gridDim.x, b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
+  const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
+  stage_item<T, MHD, BG3D, TX, TY>(A, logical);
+}
+
+// Many runs in one persistent launch: the work items of all runs form one
+// list (item_start[r] = first item of run r, item_start[nrun] = total).  Each
+// XCD takes a contiguous eighth of the list and its blocks stride through it.
+// The grid size must be a multiple of 8.
+template <typename T, bool MHD, bool BG3D, int TX, int TY>
+__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage_batch(
+    const StageArgs<T>* __restrict__ runs, const int* __restrict__ item_start, int nrun) {
+  const int total = item_start[nrun];
+  const int xcd = blockIdx.x & 7, m = blockIdx.x >> 3, nb8 = gridDim.x >> 3;
+  const int lo = (int)((long)total * xcd / 8), hi = (int)((long)total * (xcd + 1) / 8);
+  for (int item = lo + m; item < hi; item += nb8) {
+    int r = 0;
+    while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
+    stage_item<T, MHD, BG3D, TX, TY>(runs[r], item - item_start[r]);
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers (explicit instantiations over the supported tile shapes).
 // ---------------------------------------------------------------------------
@@ -462,6 +488,47 @@ static hipError_t launch_tile(int tx, int ty, const StageArgs<T>& A, hipStream_t
   if (tx == 32 && ty == 16) return launch_one<T, MHD, BG3D, 32, 16>(A, s);
   if (tx == 16 && ty == 16) return launch_one<T, MHD, BG3D, 16, 16>(A, s);
   return hipErrorInvalidValue;
+}
+
+template <typename T, bool MHD, bool BG3D, int TX, int TY>
+static hipError_t launch_batch_one(const StageArgs<T>* runs, const int* item_start, int nrun,
+                                   int nblk, hipStream_t s) {
+  hipLaunchKernelGGL((k_stage_batch<T, MHD, BG3D, TX, TY>), dim3(nblk), dim3(TX * TY), 0, s, runs,
+                     item_start, nrun);
+  return hipGetLastError();
+}
+
+template <typename T, bool MHD, bool BG3D>
+static hipError_t launch_batch_tile(int tx, int ty, const StageArgs<T>* runs, const int* item_start,
+                                    int nrun, int nblk, hipStream_t s) {
+  if (tx == 64 && ty == 4) return launch_batch_one<T, MHD, BG3D, 64, 4>(runs, item_start, nrun, nblk, s);
+  if (tx == 64 && ty == 8) return launch_batch_one<T, MHD, BG3D, 64, 8>(runs, item_start, nrun, nblk, s);
+  if (tx == 32 && ty == 8) return launch_batch_one<T, MHD, BG3D, 32, 8>(runs, item_start, nrun, nblk, s);
+  if (tx == 32 && ty == 16) return launch_batch_one<T, MHD, BG3D, 32, 16>(runs, item_start, nrun, nblk, s);
+  if (tx == 16 && ty == 16) return launch_batch_one<T, MHD, BG3D, 16, 16>(runs, item_start, nrun, nblk, s);
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_stage_batch(bool mhd, bool bg3d, int tx, int ty, const StageArgs<T>* runs,
+                              const int* item_start, int nrun, int nblk, hipStream_t s) {
+  if (mhd) return launch_batch_tile<T, true, true>(tx, ty, runs, item_start, nrun, nblk, s);
+  if (bg3d) return launch_batch_tile<T, false, true>(tx, ty, runs, item_start, nrun, nblk, s);
+  return launch_batch_tile<T, false, false>(tx, ty, runs, item_start, nrun, nblk, s);
+}
+
+template hipError_t launch_stage_batch<double>(bool, bool, int, int, const StageArgs<double>*,
+                                               const int*, int, int, hipStream_t);
+template hipError_t launch_stage_batch<float>(bool, bool, int, int, const StageArgs<float>*,
+                                              const int*, int, int, hipStream_t);
+
+// Occupancy-derived persistent grid for a batch launch (blocks resident at once).
+int stage_batch_blocks(bool mhd, int tx, int ty, bool f32, int ncu) {
+  const int waves = mhd ? 2 : 3;
+  (void)f32;
+  const int per_cu = (4 * waves * 64) / (tx * ty);
+  const int n = ncu * (per_cu > 0 ? per_cu : 1);
+  return (n + 7) / 8 * 8;
 }
 
 template <typename T>
