@@ -139,6 +139,19 @@ int sparc_hip_kernel_time(sparc_ctx* ctx, int kind, long* count, double* total_m
 /* Tile (x, y, z-chunk) the stage kernel uses and its launch size. */
 int sparc_hip_tile(sparc_ctx* ctx, int tile[3], long* nblocks);
 
+/* Set-up helper: the Alfven-limiter decimal search of initialize.f90:962-986
+ * (the search temperature T; the host then sets reduction = T/(T + red)).
+ * red = |B0|^2/(rho0 cq^2), b2 = |B0|^2, rho0: nx*ny*nz nondimensional arrays;
+ * maxva in cm/s.  Synchronises. */
+int sparc_hip_alfven_search(sparc_ctx* ctx, const double* red, const double* b2,
+                            const double* rho0, double dimb, double dimrho, double maxva,
+                            double* temp_out);
+
+/* Planes k0..k1 (0-based, inclusive) of state field `field` into out
+ * (nx*ny*(k1-k0+1) FP64, x fastest): the full-width chunk output of
+ * WRITE_OUT_CHUNK (all_modules.f90:1566-1799).  Synchronises. */
+int sparc_hip_get_slab(sparc_ctx* ctx, int field, int k0, int k1, double* out);
+
 /* Batched form: advance many independent runs (contexts on one device, one
  * precision) together.  Each RK stage is one persistent launch per kernel
  * variant and tile shape over the work items of all runs; with use_graph the

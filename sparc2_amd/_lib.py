@@ -56,7 +56,8 @@ EXPORTS = [
     "sparc_hip_filty", "sparc_hip_filtz", "sparc_hip_sync", "sparc_hip_timing",
     "sparc_hip_kernel_time", "sparc_hip_tile", "sparc_hip_strerror", "sparc_hip_last_error",
     "sparc_hip_version", "sparc_hip_batch_create", "sparc_hip_batch_advance",
-    "sparc_hip_batch_info", "sparc_hip_batch_destroy",
+    "sparc_hip_batch_info", "sparc_hip_batch_destroy", "sparc_hip_alfven_search",
+    "sparc_hip_get_slab",
 ]
 
 _lib = None
@@ -98,6 +99,9 @@ def lib() -> C.CDLL:
     L.sparc_hip_batch_advance.argtypes = [vp, i, C.POINTER(C.c_int), DP]
     L.sparc_hip_batch_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_long)]
     L.sparc_hip_batch_destroy.argtypes = [vp]
+    L.sparc_hip_alfven_search.argtypes = [vp, DP, DP, DP, C.c_double, C.c_double, C.c_double,
+                                          C.POINTER(C.c_double)]
+    L.sparc_hip_get_slab.argtypes = [vp, i, i, i, DP]
     L.sparc_hip_strerror.restype = C.c_char_p
     L.sparc_hip_strerror.argtypes = [i]
     L.sparc_hip_last_error.restype = C.c_char_p

@@ -35,6 +35,9 @@ int stage_batch_blocks(bool mhd, int tx, int ty, bool f32, int ncu);
 hipError_t launch_diag_reduce_batch(const void* jobs, int njobs, hipStream_t st);
 hipError_t launch_tick(int* c, int reset, hipStream_t st);
 hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st);
+hipError_t launch_alfven_max(const double* red, const double* b2, const double* rho0, long n,
+                             double dimb2, double T, double den0, double* part, int nblk,
+                             hipStream_t st);
 }  // namespace sparc
 
 using namespace sparc;
@@ -985,5 +988,85 @@ int sparc_hip_batch_info(sparc_batch* b, int* ngroups, long* nlaunch_blocks) {
   long n = 0;
   for (auto& g : b->groups) n += g.nblk;
   *nlaunch_blocks = n;
+  return SPARC_OK;
+}
+
+// ===========================================================================
+// Set-up helper and read-out
+// ===========================================================================
+
+// Decimal search of the Alfven limiter (initialize.f90:962-986) with the
+// maximum over the grid evaluated on the device.  red holds the ratio
+// |B0|^2/(rho0 cq^2) of initialize.f90:963, b2 = |B0|^2, all nondimensional.
+int sparc_hip_alfven_search(sparc_ctx* c, const double* red, const double* b2, const double* rho0,
+                            double dimb, double dimrho, double maxva, double* temp_out) {
+  if (!c || !red || !b2 || !rho0 || !temp_out) return fail(SPARC_E_INVALID, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  const long n = c->N;
+  double rmax = -INFINITY;
+  for (long i = 0; i < n; ++i)
+    if (red[i] > rmax) rmax = red[i];
+  if (!(rmax > 0)) {
+    *temp_out = 1.0e10;
+    return SPARC_OK;
+  }
+  double* d = nullptr;
+  const int nblk = 1024;
+  int rc = dalloc((void**)&d, sizeof(double) * (3 * n + nblk));
+  if (rc) return rc;
+  std::vector<double> part(nblk);
+  auto cleanup = [&](int code) {
+    (void)hipFree(d);
+    return code;
+  };
+  if (hipMemcpy(d, red, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + n, b2, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + 2 * n, rho0, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(SPARC_E_HIP, "alfven search upload failed"));
+  const double PI_REF = 3.14159265358979;
+  const double den0 = 4.0 * PI_REF * dimrho;
+  auto va_of = [&](double T, double* va) -> int {
+    if (launch_alfven_max(d, d + n, d + 2 * n, n, dimb * dimb, T, den0, d + 3 * n, nblk, c->st) != hipSuccess ||
+        hipMemcpyAsync(part.data(), d + 3 * n, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+      return fail(SPARC_E_HIP, "alfven search launch failed");
+    double m = -1.0e300;
+    for (double v : part)
+      if (v > m) m = v;
+    *va = m;
+    return SPARC_OK;
+  };
+  double va = 0.0, dtemp = 1.0e9, temp = -dtemp;
+  while ((dtemp > 1.0e-5) && (temp < 1.0e10 - 1.1)) {
+    while ((va < maxva) && (temp < 1.0e10)) {
+      temp = temp + dtemp;
+      if ((rc = va_of(temp, &va))) return cleanup(rc);
+    }
+    temp = temp - dtemp;
+    dtemp = dtemp / 10.0;
+    va = 0.0;
+  }
+  *temp_out = temp;
+  return cleanup(SPARC_OK);
+}
+
+// Planes k0..k1 (inclusive) of field f of the current state, FP64, x fastest
+// -- the chunk output of WRITE_OUT_CHUNK (all_modules.f90:1566-1799) for a
+// full-width slab, e.g. the photospheric vz slice (zcb = zct).
+int sparc_hip_get_slab(sparc_ctx* c, int field, int k0, int k1, double* out) {
+  if (!c || !out || field < 0 || field >= c->nv || k0 < 0 || k1 < k0 || k1 >= c->nz)
+    return fail(SPARC_E_INVALID, "bad slab request");
+  HIPCHK(hipSetDevice(c->dev));
+  const long nxy = (long)c->nx * c->ny;
+  const long n = nxy * (k1 - k0 + 1);
+  const size_t off = (size_t)field * c->fs + (size_t)k0 * nxy;
+  int rc = download(c, out, (const char*)c->pool[c->ra] + c->esz * off, n);
+  if (rc) return rc;
+  if (c->f32) {
+    std::vector<double> cc(n);
+    rc = download(c, cc.data(), (const char*)c->comp[c->rc] + 4 * off, n);
+    if (rc) return rc;
+    for (long q = 0; q < n; ++q) out[q] -= cc[q];
+  }
   return SPARC_OK;
 }

@@ -263,3 +263,38 @@ hipError_t launch_zero(void* p, long n, bool f32, hipStream_t st) {
 }
 
 }  // namespace sparc
+
+namespace sparc {
+// ---------------------------------------------------------------------------
+// Alfven limiter search (initialize.f90:966-979): one evaluation of
+// va(T) = max_p sqrt(dimb^2 T/(T+red_p) b2_p / (4 pi dimrho rho0_p)) per
+// launch, in the reference's operation order; NaN terms are skipped like
+// Fortran MAXVAL.  Blocks write partial maxima; the host finishes the max.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_alfven_max(const double* __restrict__ red,
+                                                    const double* __restrict__ b2,
+                                                    const double* __restrict__ rho0, long n,
+                                                    double dimb2, double T, double den0,
+                                                    double* __restrict__ part) {
+  __shared__ double sm[256];
+  double m = -1.0e300;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const double v = sqrt(dimb2 * T / (T + red[i]) * b2[i] / (den0 * rho0[i]));
+    if (v > m) m = v;
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o && sm[threadIdx.x + o] > sm[threadIdx.x]) sm[threadIdx.x] = sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+hipError_t launch_alfven_max(const double* red, const double* b2, const double* rho0, long n,
+                             double dimb2, double T, double den0, double* part, int nblk,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_alfven_max, dim3(nblk), dim3(256), 0, st, red, b2, rho0, n, dimb2, T, den0, part);
+  return hipGetLastError();
+}
+}  // namespace sparc

@@ -42,7 +42,11 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
   // LDS planes: 0 vx (x), 1 vy (y), 2 p (xy); MHD: 3 F3 (xy), 4 F2 (x),
   // 5 F1 (y), 6 by (x), 7 bz (xy), 8 bx (y)
   constexpr int NL = MHD ? 9 : 3;
-  __shared__ T L[NL][LY * LX];
+  // Quiet: two LDS plane buffers, the next plane's staging loads issued
+  // before the current plane's arithmetic (one barrier per plane).  MHD: one
+  // buffer (two would halve the blocks per CU), loads then barrier per plane.
+  constexpr int NB = MHD ? 1 : 2;
+  __shared__ T LB[NB][NL][LY * LX];
   __shared__ double rs[3][NT / 64 + 1];
 
   const int tid = threadIdx.x;
@@ -100,6 +104,31 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
   auto flux1 = [&](uint32_t g) -> T { return -UF(VZ, g) * boy[g] + UF(VY, g) * boz[g]; };
   auto flux2 = [&](uint32_t g) -> T { return -UF(VX, g) * boz[g] + UF(VZ, g) * box[g]; };
 
+  T(*L)[LY * LX] = LB[0];  // the plane buffer in use
+  // quiet: staging values of the next plane, held in registers
+  T pre[NS][3];
+  auto pre_load = [&](int k) {
+    const uint32_t pk = (uint32_t)k * nxy;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if (s_kind[s] == 0) continue;
+      const uint32_t g = pk + s_off[s];
+      pre[s][0] = UF(VX, g);
+      pre[s][1] = UF(VY, g);
+      pre[s][2] = UF(P, g);
+    }
+  };
+  auto pre_store = [&]() {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int kind = s_kind[s];
+      if (kind == 0) continue;
+      const int c = s_lds[s];
+      if (kind != 3) L[0][c] = pre[s][0];
+      if (kind != 2) L[1][c] = pre[s][1];
+      L[2][c] = pre[s][2];
+    }
+  };
   auto stage_plane = [&](int k) {
     const uint32_t pk = (uint32_t)k * nxy;
 #pragma unroll
@@ -199,8 +228,15 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
   const T dt = A.dt, bnext = A.beta_next;
   const uint32_t nbg = BG3D ? fs : (uint32_t)nz;
 
+  if constexpr (!MHD) pre_load(k0);
   for (int k = k0; k < k1; ++k) {
-    stage_plane(k);
+    L = LB[NB == 2 ? (k & 1) : 0];
+    if constexpr (!MHD) {
+      pre_store();
+      if (k + 1 < k1) pre_load(k + 1);
+    } else {
+      stage_plane(k);
+    }
 #pragma unroll
     for (int f = 0; f < NQ; ++f) q[f][10] = nq[f];
     fetch(k + 6, nq);
@@ -404,7 +440,8 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
         mp = fmin(mp, (double)p + (double)Buf<T>(A.p0, nbg)[gb]);
       }
     }
-    __syncthreads();
+    // single buffer: the next plane's staging must wait for these reads
+    if constexpr (NB == 1) __syncthreads();
 
 #pragma unroll
     for (int f = 0; f < NQ; ++f)

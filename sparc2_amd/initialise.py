@@ -144,12 +144,17 @@ def rk_betas(deltat: float) -> np.ndarray:
     return np.array(b) * deltat
 
 
-def _reduction_search(bg, dimb, dimrho, maxva):
-    """Decimal search of the Alfven limiter, initialize.f90:962-986."""
+def _reduction_search(bg, dimb, dimrho, maxva, ops=None):
+    """Decimal search of the Alfven limiter, initialize.f90:962-986.  With an
+    operator object that offers ``alfven_search`` (the GPU solver) the
+    maximum over the grid is evaluated on the device."""
     box, boy, boz, rho0 = bg["box"], bg["boy"], bg["boz"], bg["rho0"]
     cq = bg["cq"][:, None, None]
     red = (boz * boz + box * box + boy * boy) / (rho0 * (cq * cq))
     b2 = box * box + boy * boy + boz * boz
+    if ops is not None and hasattr(ops, "alfven_search"):
+        temp = ops.alfven_search(red, b2, rho0, dimb, dimrho, maxva)
+        return temp / (temp + red)
     den = 4.0 * PI * dimrho * rho0
     va, dtemp = 0.0, 1.0e9
     temp = -dtemp
@@ -203,7 +208,7 @@ def initialise_rhs(st: SetupState, ops) -> None:
         sponge = np.ascontiguousarray(sponge)
 
     if cfg.mhd:
-        bg["reduction"] = _reduction_search(bg, st.dimb, st.dimrho, cfg.maxva)
+        bg["reduction"] = _reduction_search(bg, st.dimb, st.dimrho, cfg.maxva, ops)
         t = ops.ddy(bg["box"], bc)
         bg["curlboz"] = ops.ddx(bg["boy"], bc) - t
         t = ops.ddy(bg["boz"], bc)

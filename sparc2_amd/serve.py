@@ -1,0 +1,225 @@
+"""Served form: forward-simulation requests with continuous batching.
+
+A request is one SPARC2 run (grid, background, source position, step count).
+The server admits arrived requests in arrival order while their device
+footprint fits the HBM budget, keeps every admitted run's state resident, and
+advances all admitted runs together with the batched kernels.  Between
+batched advances it returns each run's photospheric vz slice every 45 s of
+simulated time (the full-width chunk output of WRITE_OUT_CHUNK,
+all_modules.f90:1566-1799, at the plane nearest z = 0), retires finished runs
+and admits waiting ones -- requests that arrive while others are running join
+at the next advance.  Latency is measured from arrival to completion.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from .background import make_background
+from .config import RunConfig
+from .initialise import prepare
+
+VZ = 3
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float                 # s after the stream start
+    cfg: RunConfig
+    zrange: tuple                  # (zbot, ztop) in Mm
+    nsteps: int
+    B0: float = 0.0                # G; 0 = quiet
+    slice_every_s: float = 45.0
+
+    def build_background(self):
+        if self.cfg.magnetic == 2:
+            return make_background(self.cfg, *self.zrange, B0=self.B0, R0=4.0e8)
+        return make_background(self.cfg, *self.zrange)
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float = math.nan
+    finished: float = math.nan
+    slices: List = dataclasses.field(default_factory=list)   # (time index, 2-D vz)
+    final: Optional[np.ndarray] = None
+    points: int = 0
+    steps: int = 0
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def device_footprint(cfg: RunConfig) -> int:
+    """Bytes a run keeps on the device: 3 state buffers (+2 compensation
+    buffers in FP32), backgrounds, CPML arrays."""
+    w = 4 if cfg.precision == "fp32" else 8
+    fs = -(-cfg.npoints // 256) * 256
+    n_state = 3 + (2 if cfg.precision == "fp32" else 0)
+    n_bg = 18 if cfg.mhd else 7
+    psi = sum(int(np.prod(cfg.face_shape(f))) for f in range(6) if cfg.face_active(f))
+    return w * (n_state * cfg.nv * fs + n_bg * fs + psi * (3 + 2 * cfg.npsi)) + (1 << 20)
+
+
+def surface_plane(zaxis_mm) -> int:
+    return int(np.argmin(np.abs(np.asarray(zaxis_mm))))
+
+
+def cfl_timestep(cfg: RunConfig, bg, vmax_alfven=1.0e7, courant=1.0, quantum=0.05) -> float:
+    """Time step from the reference's CFL estimate (driver.f90:83-117): the
+    smallest vertical crossing time of the fastest wave, with the Alfven speed
+    capped by the limiter, rounded down to `quantum` seconds."""
+    z = np.asarray(bg["zaxis"]) * 1e8
+    dz = np.diff(z)
+    c = np.asarray(bg["cs0"])
+    vmax = c.max(axis=(1, 2))
+    if cfg.magnetic == 2:
+        vmax = np.sqrt(vmax ** 2 + vmax_alfven ** 2)
+    t = courant * float(np.min(dz / np.maximum(vmax[:-1], vmax[1:])))
+    return max(quantum, math.floor(t / quantum) * quantum)
+
+
+def poisson_stream(n: int, rate: float = 4.0, seed: int = 7, precision: str = "fp32",
+                   grids=((64, 64, 64), (128, 128, 128), (192, 192, 160), (256, 256, 192),
+                          (384, 384, 256)),
+                   steps=(500, 5000)) -> List[Request]:
+    """Seeded request stream: Poisson arrivals, grids drawn from `grids`,
+    B0 uniform in [0, 3] kG (0 -> quiet), random source position in the
+    central half, step count uniform in `steps`."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate)
+        nx, ny, nz = grids[rng.integers(len(grids))]
+        B0 = float(rng.uniform(0.0, 3000.0))
+        mhd = B0 > 1.0
+        L = 100.0e8 if mhd else 50.0e8
+        px, py = rng.uniform(0.25 * L, 0.75 * L, size=2)
+        zr = (-20.0, 1.0) if mhd else (-10.0, 0.5)
+        nsteps = int(rng.integers(steps[0], steps[1] + 1))
+        cfg = RunConfig(
+            nx=nx, ny=ny, nz=nz, xlength=L, ylength=L, timestep=1.0, magnetic=2 if mhd else 0,
+            periodic=not mhd, use_pml=True, use_hpml=mhd,
+            npml=(10, 10, 10, 10, 10, 10) if mhd else (0, 0, 0, 0, 10, 10),
+            pmlc=((9.2103, 2.0, 1.0, 0.1),) * 6, t_xy=30.0, t_z=10.0,
+            pulse_t0=1000.0 / 3.0, pulse_t1=450.0, pulse_st=150.0,
+            pulse_px=float(px), pulse_py=float(py), pulse_pz=-0.15e8,
+            pulse_sr=1.0e8, pulse_sz=1.0e8, pulse_amp=1.0e4, maxva=1.0e7, precision=precision)
+        out.append(Request(rid=rid, arrival=t, cfg=cfg, zrange=zr, nsteps=nsteps, B0=B0))
+    return out
+
+
+class Server:
+    def __init__(self, hbm_budget: float = 200e9, max_chunk: int = 200, use_graph: bool = True,
+                 device: int = 0, keep_final: bool = False,
+                 clock: Callable[[], float] = time.perf_counter):
+        self.budget = hbm_budget
+        self.max_chunk = max_chunk
+        self.use_graph = use_graph
+        self.device = device
+        self.keep_final = keep_final
+        self.clock = clock
+
+    def _start(self, req: Request):
+        from .solver import Solver
+        bg = req.build_background()
+        cfg = req.cfg
+        cfg.timestep = cfl_timestep(cfg, bg)
+        s = Solver(cfg, bg, device=self.device)
+        every = max(1, int(round(req.slice_every_s / cfg.timestep)))
+        return {"solver": s, "done": 0, "every": every, "k0": surface_plane(bg["zaxis"])}
+
+    def run(self, requests: List[Request], verbose: bool = False) -> Dict:
+        from .batch import Batch
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        results = {r.rid: Result(rid=r.rid, arrival=r.arrival) for r in reqs}
+        waiting: List[Request] = []
+        active: Dict[int, dict] = {}
+        used = 0.0
+        t0 = self.clock()
+        now = lambda: self.clock() - t0
+        nxt = 0
+        batch, members = None, ()
+        busy = 0.0
+        while nxt < len(reqs) or waiting or active:
+            while nxt < len(reqs) and reqs[nxt].arrival <= now():
+                waiting.append(reqs[nxt])
+                nxt += 1
+            # admission in arrival order while the footprint fits
+            while waiting:
+                fp = device_footprint(waiting[0].cfg)
+                if active and used + fp > self.budget:
+                    break
+                req = waiting.pop(0)
+                st = self._start(req)
+                st["req"], st["fp"] = req, fp
+                active[req.rid] = st
+                used += fp
+                results[req.rid].admitted = now()
+                results[req.rid].points = req.cfg.npoints
+                if verbose:
+                    print(f"[serve] t={now():7.2f}s admit {req.rid} {req.cfg.nx}x{req.cfg.ny}x{req.cfg.nz} "
+                          f"{'MHD' if req.cfg.mhd else 'quiet'} {req.nsteps} steps", flush=True)
+            if not active:
+                if nxt < len(reqs):
+                    time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
+                continue
+            key = tuple(sorted(active))
+            if key != members:
+                if batch is not None:
+                    batch.close()
+                sol = [active[r]["solver"] for r in key]
+                batch = Batch(sol, use_graph=self.use_graph)
+                batch.time = [active[r]["done"] for r in key]
+                members = key
+            # advance to the next event of any run: a slice or its end
+            chunk = self.max_chunk
+            for r in key:
+                a = active[r]
+                left = a["req"].nsteps - a["done"]
+                to_slice = a["every"] - (a["done"] % a["every"])
+                chunk = min(chunk, left, to_slice)
+            tb = self.clock()
+            batch.advance(chunk)
+            busy += self.clock() - tb
+            for r in key:
+                a = active[r]
+                a["done"] += chunk
+                res = results[r]
+                res.steps = a["done"]
+                if a["done"] % a["every"] == 0:
+                    res.slices.append((a["done"], a["solver"].get_slab(VZ, a["k0"], a["k0"])[0]))
+                if a["done"] >= a["req"].nsteps:
+                    if self.keep_final:
+                        res.final = a["solver"].get_state()
+                    res.finished = now()
+                    a["solver"].close()
+                    used -= a["fp"]
+                    del active[r]
+                    if verbose:
+                        print(f"[serve] t={now():7.2f}s done  {r} latency {res.latency:.2f}s", flush=True)
+        if batch is not None:
+            batch.close()
+        wall = now()
+        lat = np.array([results[r.rid].latency for r in reqs])
+        pts = sum(results[r.rid].points * results[r.rid].steps for r in reqs)
+        return {
+            "results": results,
+            "wall_s": wall,
+            "busy_s": busy,
+            "runs_per_h": len(reqs) / wall * 3600.0,
+            "gpts": pts / wall / 1e9,
+            "gpts_busy": pts / busy / 1e9 if busy > 0 else float("nan"),
+            "latency_p50": float(np.percentile(lat, 50)),
+            "latency_p95": float(np.percentile(lat, 95)),
+            "latency_p99": float(np.percentile(lat, 99)),
+        }

@@ -202,6 +202,23 @@ class Solver:
     def filtz(self, x):
         return self._op(lib().sparc_hip_filtz, x)
 
+    def alfven_search(self, red, b2, rho0, dimb, dimrho, maxva) -> float:
+        """Search temperature of the Alfven limiter, maximum over the grid on
+        the device (initialize.f90:962-986)."""
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (red, b2, rho0)]
+        t = C.c_double()
+        check(lib().sparc_hip_alfven_search(self.h, *(dptr(v) for v in arrs), float(dimb),
+                                            float(dimrho), float(maxva), C.byref(t)),
+              "sparc_hip_alfven_search")
+        return t.value
+
+    def get_slab(self, field: int, k0: int, k1: int) -> np.ndarray:
+        """Planes k0..k1 of one state field (the chunk output of WRITE_OUT_CHUNK)."""
+        out = np.zeros((k1 - k0 + 1, self.cfg.ny, self.cfg.nx))
+        check(lib().sparc_hip_get_slab(self.h, int(field), int(k0), int(k1), dptr(out)),
+              "sparc_hip_get_slab")
+        return out
+
     # ----------------------------------------------------------- measurement
     def timing(self, enable: bool):
         check(lib().sparc_hip_timing(self.h, int(enable)), "sparc_hip_timing")

@@ -1,0 +1,41 @@
+"""Served form: requests arriving while others run join the batch; each
+result (photospheric vz slices and final state) is the same as the request
+served alone (bitwise: the batched launch runs the same per-tile code)."""
+import copy
+
+import numpy as np
+import pytest
+
+from tests.cases import mhd_cfg, quiet_cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    from sparc2_amd.serve import Request
+    specs = [
+        (0.00, quiet_cfg(nx=32, ny=32, nz=40, precision="fp32"), (-10.0, 0.5), 140, 0.0),
+        (0.02, mhd_cfg(nx=32, ny=32, nz=40, precision="fp32"), (-5.0, 1.0), 90, 2500.0),
+        (0.04, quiet_cfg(nx=48, ny=32, nz=48, pulse_px=20e8, precision="fp32"), (-10.0, 0.5), 60, 0.0),
+        (0.30, mhd_cfg(nx=48, ny=32, nz=40, periodic=True, precision="fp32"), (-5.0, 1.0), 75, 1500.0),
+    ]
+    return [Request(rid=q, arrival=t, cfg=c, zrange=z, nsteps=n, B0=b, slice_every_s=15.0)
+            for q, (t, c, z, n, b) in enumerate(specs)]
+
+
+def test_served_results_equal_alone():
+    from sparc2_amd.serve import Server
+    reqs = _requests()
+    mixed = Server(max_chunk=25, keep_final=True).run(copy.deepcopy(reqs), verbose=True)
+    for r in reqs:
+        alone = Server(max_chunk=25, keep_final=True).run([copy.deepcopy(r)])
+        a = mixed["results"][r.rid]
+        b = alone["results"][r.rid]
+        assert a.steps == b.steps == r.nsteps
+        assert len(a.slices) == len(b.slices) > 0
+        for (ta, sa), (tb, sb) in zip(a.slices, b.slices):
+            assert ta == tb
+            np.testing.assert_array_equal(sa, sb)
+        np.testing.assert_array_equal(a.final, b.final)
+    print({k: v for k, v in mixed.items() if k != "results"}, flush=True)
+    assert mixed["latency_p99"] >= mixed["latency_p50"] > 0
