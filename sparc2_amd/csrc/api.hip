@@ -284,6 +284,12 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
   c->nz = q.nz;
   c->N = (long)q.nx * q.ny * q.nz;
   c->fs = round_up(c->N, 256);
+  // kernels address one field slab (and one CPML face array) with 32-bit
+  // byte offsets through buffer descriptors
+  if ((unsigned long long)c->fs * c->esz >= (1ull << 32)) {
+    delete c;
+    return fail(SPARC_E_INVALID, "one field slab must stay below 4 GiB: split the domain in z");
+  }
   choose_tile(c);
   int rc = check_tile(c);
   if (rc) {
@@ -528,8 +534,10 @@ static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
     D.pst = (T*)F.pst;
   }
   A.partial = c->partial;
-  A.comp = c->f32 ? (T*)c->comp[c->rc] : nullptr;
-  A.a = (T*)c->pool[c->ra];
+  for (int f = 0; f < 8; ++f) {
+    A.comp[f] = c->f32 ? (T*)c->comp[c->rc] + (size_t)f * c->fs : nullptr;
+    A.a[f] = (T*)c->pool[c->ra] + (size_t)f * c->fs;
+  }
 }
 
 // Stage s of LDDRK-5: u_1 = a; stage s reads u_s and writes
@@ -541,8 +549,10 @@ static void set_stage(sparc_ctx* c, StageArgs<T>& A, int s) {
   A.first = s == 1;
   A.last = s == 5;
   A.beta_next = A.last ? (T)0 : (T)c->cfg.betas[s];
-  A.u = (const T*)c->pool[uin];
-  A.un = A.last ? nullptr : (T*)c->pool[uout];
+  for (int f = 0; f < 8; ++f) {
+    A.u[f] = (const T*)c->pool[uin] + (size_t)f * c->fs;
+    A.un[f] = A.last ? nullptr : (T*)c->pool[uout] + (size_t)f * c->fs;
+  }
 }
 
 template <typename T>

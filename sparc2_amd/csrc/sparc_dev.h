@@ -98,6 +98,7 @@ template <typename T> struct Face {
 // base (read at the point only).  A middle stage writes the next stage value
 // a + beta_next*K into un; the last stage overwrites a with a + dt*K
 // (step.f90:58) -- safe in place because a is only read at the point.
+// State arrays are given per field (the slab base of each field).
 template <typename T> struct StageArgs {
   int nx, ny, nz, kc;
   long fs;
@@ -108,10 +109,10 @@ template <typename T> struct StageArgs {
   // table and step counter, so a captured step replays unchanged)
   const T* amp_tab;
   const int* tstep;
-  const T* u;
-  T* a;
-  T* un;
-  T* comp;  // Kahan compensation (fp32 only), updated in place on the last stage
+  const T* u[8];
+  T* a[8];
+  T* un[8];
+  T* comp[8];  // Kahan compensation (fp32 only), updated in place on the last stage
   // backgrounds: 3-D (bg3d) or nz-long z profiles
   const T *rho0, *rhoinv, *c2rho0, *gr0z, *gp0z, *sponge, *p0;
   // MHD backgrounds (always 3-D)
@@ -121,6 +122,17 @@ template <typename T> struct StageArgs {
   double* partial;  // last stage: 3 doubles per block (KE sum, min rho, min p)
 };
 
+// The stage kernels read their arguments through a pointer into the constant
+// address space (the kernel-argument segment, or a device array that no
+// kernel writes while it runs), so uniform fields become scalar loads.
+#define SPARC_AS4 __attribute__((address_space(4)))
+template <typename T> using CArgs = const SPARC_AS4 StageArgs<T>*;
+template <typename T> using CPtr = const SPARC_AS4 T*;
+// Opaque redefinition of the argument pointer: values derived from it
+// (array bases, buffer descriptors) are reloaded after this point instead of
+// being held in scalar registers across a loop.
+#define SPARC_REFRESH(p) asm volatile("" : "+s"(p))
+
 // One run's diagnostic reduction in the batched form.
 struct DiagJob {
   const double* partial;
@@ -128,15 +140,6 @@ struct DiagJob {
   int nblk;
   int pad_;
 };
-
-// PML update of one derivative family at one point (RHS.f90:52-55): the psi
-// RHS comes from the raw derivative, then the derivative is replaced by
-// d/kappa + psi.  The next psi stage value (step.f90:34-36) or, on the last
-// stage, the advanced base (step.f90:59-61) is written in place.
-template <typename T> struct Buf;
-template <typename T>
-__device__ __forceinline__ void pml_family(const Face<T>& F, uint32_t fp, int q, T& d,
-                                           bool first, bool last, T beta_next, T dt);
 
 // ---------------------------------------------------------------------------
 // Raw buffer view of one array: a uniform descriptor (SGPRs) plus a 32-bit
@@ -167,9 +170,14 @@ template <typename T> struct Buf {
   }
 };
 
-template <typename T>
-__device__ __forceinline__ void pml_family(const Face<T>& F, uint32_t fp, int q, T& d,
-                                           bool first, bool last, T beta_next, T dt) {
+// PML update of one derivative family at one point (RHS.f90:52-55): the psi
+// RHS comes from the raw derivative, then the derivative is replaced by
+// d/kappa + psi.  The next psi stage value (step.f90:34-36) or, on the last
+// stage, the advanced base (step.f90:59-61) is written in place.  FR is a
+// Face<T> in any address space.
+template <typename T, typename FR>
+__device__ __forceinline__ void pml_family(FR& F, uint32_t fp, int q, T& d, bool first, bool last,
+                                           T beta_next, T dt) {
   const uint32_t n = (uint32_t)F.fsize;
   const Buf<T> psi(F.psi + (size_t)q * n, n), pst(F.pst + (size_t)q * n, n);
   const Buf<T> fa(F.a, n), fb(F.b, n), ki(F.kinv, n);

@@ -30,9 +30,9 @@ constexpr int stage_waves() {
 }
 
 // One work item: tile `tile` of z chunk `chunk` (logical = chunk*nblk_xy + tile)
-// of the run described by A.
+// of the run whose arguments Ap points to.
 template <typename T, bool MHD, bool BG3D, int TX, int TY>
-__device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logical) {
+__device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   constexpr int NT = TX * TY;
   constexpr int LX = TX + 10, LY = TY + 10;
   constexpr int NR = TY * LX + 10 * TX;  // cross-shaped staging region
@@ -51,15 +51,16 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const int nx = A.nx, ny = A.ny, nz = A.nz;
-  const uint32_t fs = (uint32_t)A.fs;
+  const int nx = Ap->nx, ny = Ap->ny, nz = Ap->nz;
+  const uint32_t fs = (uint32_t)Ap->fs;
   const uint32_t nxy = (uint32_t)nx * (uint32_t)ny;
-  const bool first = A.first, last = A.last, periodic = A.periodic;
+  const bool first = Ap->first, last = Ap->last, periodic = Ap->periodic;
 
-  const int tile = logical % A.nblk_xy, chunk = logical / A.nblk_xy;
-  const int i0 = (tile % A.ntx) * TX, j0 = (tile / A.ntx) * TY;
-  const int k0 = chunk * A.kc;
-  const int k1 = min(k0 + A.kc, nz);
+  const int nblk_xy = Ap->nblk_xy, ntx = Ap->ntx, kc = Ap->kc;
+  const int tile = logical % nblk_xy, chunk = logical / nblk_xy;
+  const int i0 = (tile % ntx) * TX, j0 = (tile / ntx) * TY;
+  const int k0 = chunk * kc;
+  const int k1 = min(k0 + kc, nz);
   const int i = i0 + tx, j = j0 + ty;
   const bool active = (i < nx) && (j < ny);
   const uint32_t ij = active ? (uint32_t)(i + nx * j) : 0u;
@@ -95,14 +96,12 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
     s_lds[s] = row * LX + col;
   }
 
-  // Buffer views.  The lambdas below capture only locals: capturing the
-  // kernel argument struct would force a private copy of it.
-  const T* const u = A.u;
-  const Buf<T> box(A.box, fs), boy(A.boy, fs), boz(A.boz, fs);
-  const T sxs = A.sxs, sys = A.sys;
-  auto UF = [&](int f, uint32_t g) -> T { return Buf<T>(u + (size_t)f * fs, fs)[g]; };
-  auto flux1 = [&](uint32_t g) -> T { return -UF(VZ, g) * boy[g] + UF(VY, g) * boz[g]; };
-  auto flux2 = [&](uint32_t g) -> T { return -UF(VX, g) * boz[g] + UF(VZ, g) * box[g]; };
+  // Array views, rebuilt from the argument pointer where they are used.
+  const T sxs = Ap->sxs, sys = Ap->sys;
+  auto UF = [&](int f, uint32_t g) -> T { return Buf<T>(Ap->u[f], fs)[g]; };
+  auto B3 = [&](const T* p, uint32_t g) -> T { return Buf<T>(p, fs)[g]; };
+  auto flux1 = [&](uint32_t g) -> T { return -UF(VZ, g) * B3(Ap->boy, g) + UF(VY, g) * B3(Ap->boz, g); };
+  auto flux2 = [&](uint32_t g) -> T { return -UF(VX, g) * B3(Ap->boz, g) + UF(VZ, g) * B3(Ap->box, g); };
 
   T(*L)[LY * LX] = LB[0];  // the plane buffer in use
   // quiet: staging values of the next plane, held in registers
@@ -144,7 +143,7 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
       L[2][c] = p;
       if constexpr (MHD) {
         const T vz = UF(VZ, g), bx = UF(BX, g), by = UF(BY, g), bz = UF(BZ, g);
-        const T b0x = box[g], b0y = boy[g], b0z = boz[g];
+        const T b0x = B3(Ap->box, g), b0y = B3(Ap->boy, g), b0z = B3(Ap->boz, g);
         L[3][c] = -vy * b0x + vx * b0y;           // F3 = -vy box + vx boy
         if (wx) L[4][c] = -vx * b0z + vz * b0x;   // F2
         if (wy) L[5][c] = -vz * b0y + vy * b0z;   // F1
@@ -207,8 +206,8 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
         dst[2] = UF(BX, g);
         dst[3] = UF(BY, g);
         const T vx = UF(VX, g), vy = UF(VY, g), vz = dst[0];
-        dst[4] = -vz * boy[g] + vy * boz[g];
-        dst[5] = -vx * boz[g] + vz * box[g];
+        dst[4] = -vz * B3(Ap->boy, g) + vy * B3(Ap->boz, g);
+        dst[5] = -vx * B3(Ap->boz, g) + vz * B3(Ap->box, g);
       }
     } else {
 #pragma unroll
@@ -225,11 +224,16 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
   fetch(k0 + 5, nq);
 
   double ke = 0.0, mr = 1.0e300, mp = 1.0e300;
-  const T dt = A.dt, bnext = A.beta_next;
-  const uint32_t nbg = BG3D ? fs : (uint32_t)nz;
+  const T dt = Ap->dt, bnext = Ap->beta_next;
+  // background at the point: 3-D array, or the z profile (uniform, scalar load)
+  auto BG = [&](const T* p, uint32_t g, int k) -> T {
+    if constexpr (BG3D) return Buf<T>(p, fs)[g];
+    else return ((CPtr<T>)p)[k];
+  };
 
   if constexpr (!MHD) pre_load(k0);
   for (int k = k0; k < k1; ++k) {
+    SPARC_REFRESH(Ap);
     L = LB[NB == 2 ? (k & 1) : 0];
     if constexpr (!MHD) {
       pre_store();
@@ -244,7 +248,7 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
     // ---- z derivatives (derivatives.f90:481-487,509-518,536-545) ----
     T dz[NQ];
     if (active) {
-      const T st = A.stretch[k];
+      const T st = ((CPtr<T>)Ap->stretch)[k];
       if (k < nz - 5) {
 #pragma unroll
         for (int f = 0; f < NQ; ++f) {
@@ -276,10 +280,10 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
       }
     }
     __syncthreads();
+    SPARC_REFRESH(Ap);
 
     if (active) {
       const uint32_t g = ij + (uint32_t)k * nxy;
-      const uint32_t gb = BG3D ? g : (uint32_t)k;
       // xy derivatives and centre values from LDS
       T dvxdx = dx_lds(0), dvydy = dy_lds(1), gpx = dx_lds(2), gpy = dy_lds(2);
       const int cc = (ty + 5) * LX + tx + 5;
@@ -307,7 +311,7 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
       // ---- CPML (RHS.f90:50-93, 424-546) ----
 #pragma unroll
       for (int fz = ZBOT; fz <= ZTOP; ++fz) {
-        const Face<T>& F = A.face[fz];
+        const auto& F = Ap->face[fz];
         const int kp = k - F.off;
         if (!F.active || kp < 0 || kp >= F.n) continue;
         const uint32_t fp = ij + nxy * (uint32_t)kp;
@@ -322,7 +326,7 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
       }
 #pragma unroll
       for (int fx = XBOT; fx <= XTOP; ++fx) {
-        const Face<T>& F = A.face[fx];
+        const auto& F = Ap->face[fx];
         const int ip = i - F.off;
         if (!F.active || ip < 0 || ip >= F.n) continue;
         const uint32_t fp = (uint32_t)(ip + F.n * (j + ny * k));
@@ -337,7 +341,7 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
       }
 #pragma unroll
       for (int fy = YBOT; fy <= YTOP; ++fy) {
-        const Face<T>& F = A.face[fy];
+        const auto& F = Ap->face[fy];
         const int jp = j - F.off;
         if (!F.active || jp < 0 || jp >= F.n) continue;
         const uint32_t fp = (uint32_t)(i + nx * (jp + F.n * k));
@@ -352,48 +356,47 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
       }
 
       // ---- pointwise RHS ----
-      const Buf<T> brho0(A.rho0, nbg), brhoinv(A.rhoinv, nbg), bsp(A.sponge, nbg);
-      const T rho0 = brho0[gb], rhoinv = brhoinv[gb], sp = bsp[gb];
+      const T rho0 = BG(Ap->rho0, g, k), rhoinv = BG(Ap->rhoinv, g, k), sp = BG(Ap->sponge, g, k);
       const T div = dvxdx + dvydy + dvzdz;
       T K[8];
       if constexpr (!MHD) {
-        const Buf<T> bgr0z(A.gr0z, nbg), bc2r(A.c2rho0, nbg), bgp0z(A.gp0z, nbg);
+        const T gk = ((CPtr<T>)Ap->g)[k];
         // RHS.f90:95-101,136,152
-        K[RHO] = -bgr0z[gb] * vz - rho0 * div - sp * rho;
+        K[RHO] = -BG(Ap->gr0z, g, k) * vz - rho0 * div - sp * rho;
         K[VX] = -rhoinv * gpx - sp * vx;
         K[VY] = -rhoinv * gpy - sp * vy;
-        K[VZ] = -rhoinv * (gpz + rho * A.g[k]) - vz * sp;
-        K[P] = -bc2r[gb] * div - vz * bgp0z[gb] - sp * p;
+        K[VZ] = -rhoinv * (gpz + rho * gk) - vz * sp;
+        K[P] = -BG(Ap->c2rho0, g, k) * div - vz * BG(Ap->gp0z, g, k) - sp * p;
       } else {
-        const Buf<T> bred(A.red, fs), bcbox(A.cbox, fs), bcboy(A.cboy, fs), bcboz(A.cboz, fs);
-        const Buf<T> bgr0x(A.gr0x, fs), bgr0y(A.gr0y, fs), bgr0z(A.gr0z, fs);
-        const Buf<T> bgp0x(A.gp0x, fs), bgp0y(A.gp0y, fs), bgp0z(A.gp0z, fs), bc2r(A.c2rho0, fs);
-        const T red = bred[g];
-        const T b0x = box[g], b0y = boy[g], b0z = boz[g];
+        const T red = B3(Ap->red, g);
+        const T b0x = B3(Ap->box, g), b0y = B3(Ap->boy, g), b0z = B3(Ap->boz, g);
         const T bx = q[2][5], by = q[3][5];
         // RHS.f90:548-580
         K[BX] = red * (dyf3 - dzf2);
         K[BY] = red * (dzf1 - dxf3);
         K[BZ] = red * (dxf2 - dyf1);
         const T cbx = dybz - dzby, cby = -dxbz + dzbx, cbz = dxby - dybx;
-        K[RHO] = -bgr0x[g] * vx - bgr0y[g] * vy - bgr0z[g] * vz - rho0 * div - sp * rho;
-        K[P] = -bc2r[g] * div - vx * bgp0x[g] - vy * bgp0y[g] - vz * bgp0z[g] - sp * p;
+        K[RHO] = -B3(Ap->gr0x, g) * vx - B3(Ap->gr0y, g) * vy - B3(Ap->gr0z, g) * vz - rho0 * div -
+                 sp * rho;
+        K[P] = -B3(Ap->c2rho0, g) * div - vx * B3(Ap->gp0x, g) - vy * B3(Ap->gp0y, g) -
+               vz * B3(Ap->gp0z, g) - sp * p;
         // CROSS(curl b, B0) and CROSS(curl B0, b), derivatives.f90:646-655
         const T c1 = cby * b0z - cbz * b0y, c2 = b0x * cbz - cbx * b0z, c3 = cbx * b0y - cby * b0x;
-        const T cox = bcbox[g], coy = bcboy[g], coz = bcboz[g];
+        const T cox = B3(Ap->cbox, g), coy = B3(Ap->cboy, g), coz = B3(Ap->cboz, g);
         const T e1 = coy * bz - coz * by;
         const T e2 = bx * coz - cox * bz;
         const T e3 = cox * by - coy * bx;
         K[VX] = -rhoinv * (gpx - c1 * red) - sp * vx + rhoinv * e1 * red;
         K[VY] = -rhoinv * (gpy - c2 * red) - sp * vy + rhoinv * e2 * red;
-        K[VZ] = -rhoinv * (gpz - c3 * red) - sp * vz + rhoinv * e3 * red - rhoinv * rho * A.g[k];
+        K[VZ] = -rhoinv * (gpz - c3 * red) - sp * vz + rhoinv * e3 * red - rhoinv * rho * ((CPtr<T>)Ap->g)[k];
       }
       // pulse source (driver.f90:137-148 separable form; RHS.f90:138-145,632-639)
-      if (A.pulse_dir >= 1 && A.pulse_dir <= 3) {
-        const T amp = A.amp_tab ? A.amp_tab[*A.tstep] : A.src_amp;
-        const T s = amp * A.sx[i] * A.sy[j] * A.sz[k];
-        if (A.pulse_dir == 1) K[VX] += s;
-        else if (A.pulse_dir == 2) K[VY] += s;
+      const int pdir = Ap->pulse_dir;
+      if (pdir >= 1 && pdir <= 3) {
+        const T amp = Ap->amp_tab ? Ap->amp_tab[*Ap->tstep] : Ap->src_amp;
+        const T s = amp * Buf<T>(Ap->sx, nx)[i] * Buf<T>(Ap->sy, ny)[j] * ((CPtr<T>)Ap->sz)[k];
+        if (pdir == 1) K[VX] += s;
+        else if (pdir == 2) K[VY] += s;
         else K[VZ] += s;
       }
       // boundary-plane zeroing (RHS.f90:154-164,647-657)
@@ -403,24 +406,22 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
 #pragma unroll
         for (int f = 0; f < NV; ++f) K[f] = 0;
       }
-      T* const ap = A.a;
       if (!last) {
         // next stage value, step.f90:33
-        T* const un = A.un;
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
-          const Buf<T> af(ap + (size_t)f * fs, fs), uf(un + (size_t)f * fs, fs);
+          const Buf<T> af(Ap->a[f], fs), uf(Ap->un[f], fs);
           uf.st(g, af[g] + bnext * K[f]);
         }
       } else {
         T an[NV];
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
-          const Buf<T> af(ap + (size_t)f * fs, fs);
+          const Buf<T> af(Ap->a[f], fs);
           const T a0 = af[g];
           if constexpr (sizeof(T) == 4) {
             // compensated update: a - c carries the bits a + dt K loses
-            const Buf<T> cf(A.comp + (size_t)f * fs, fs);
+            const Buf<T> cf(Ap->comp[f], fs);
             const T c0 = cf[g];
             const T y = dt * K[f] - c0;
             const T t = a0 + y;
@@ -437,7 +438,7 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
                          2.0;
         ke += m * m;
         mr = fmin(mr, (double)rho + (double)rho0);
-        mp = fmin(mp, (double)p + (double)Buf<T>(A.p0, nbg)[gb]);
+        mp = fmin(mp, (double)p + (double)BG(Ap->p0, g, k));
       }
     }
     // single buffer: the next plane's staging must wait for these reads
@@ -471,9 +472,10 @@ __device__ __forceinline__ void stage_item(const StageArgs<T>& A, const int logi
         r = fmin(r, rs[1][q2]);
         pp = fmin(pp, rs[2][q2]);
       }
-      A.partial[3 * (long)logical + 0] = s;
-      A.partial[3 * (long)logical + 1] = r;
-      A.partial[3 * (long)logical + 2] = pp;
+      double* const part = Ap->partial;
+      part[3 * (long)logical + 0] = s;
+      part[3 * (long)logical + 1] = r;
+      part[3 * (long)logical + 2] = pp;
     }
   }
 }
@@ -486,7 +488,8 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage(const
   const int nblk = gridDim.x, b = blockIdx.x;
   const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
   const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
-  stage_item<T, MHD, BG3D, TX, TY>(A, logical);
+  // A sits at offset 0 of the kernel-argument segment
+  stage_item<T, MHD, BG3D, TX, TY>((CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr(), logical);
 }
 
 // Many runs in one persistent launch: the work items of all runs form one
@@ -502,7 +505,8 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage_batch
   for (int item = lo + m; item < hi; item += nb8) {
     int r = 0;
     while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
-    stage_item<T, MHD, BG3D, TX, TY>(runs[r], item - item_start[r]);
+    // no kernel writes the argument array while the batch runs
+    stage_item<T, MHD, BG3D, TX, TY>((CArgs<T>)(runs + r), item - item_start[r]);
     __syncthreads();
   }
 }
