@@ -186,6 +186,56 @@ module sparc_hip
       integer(c_long), intent(out) :: nblocks
     end function
 
+    ! Alfven-limiter search temperature (initialize.f90:962-986)
+    integer(c_int) function sparc_hip_alfven_search(ctx, red, b2, rho0, dimb, dimrho, maxva, temp) &
+         bind(C, name="sparc_hip_alfven_search")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: red(*), b2(*), rho0(*)
+      real(c_double), value :: dimb, dimrho, maxva
+      real(c_double), intent(out) :: temp
+    end function
+
+    ! planes k0..k1 (0-based) of one state field: the WRITE_OUT_CHUNK read-out
+    integer(c_int) function sparc_hip_get_slab(ctx, field, k0, k1, out) bind(C, name="sparc_hip_get_slab")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field, k0, k1
+      real(c_double), intent(out) :: out(*)
+    end function
+
+    ! batched form: runs(nrun) are contexts created on one device
+    integer(c_int) function sparc_hip_batch_create(runs, nrun, use_graph, batch) &
+         bind(C, name="sparc_hip_batch_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: runs(*)
+      integer(c_int), value :: nrun, use_graph
+      type(c_ptr), intent(out) :: batch
+    end function
+
+    ! amps(nsteps, nrun): amps(n, r) is run r's source amplitude at step n
+    integer(c_int) function sparc_hip_batch_advance(batch, nsteps, time0, amps) &
+         bind(C, name="sparc_hip_batch_advance")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int), value :: nsteps
+      integer(c_int), intent(in) :: time0(*)
+      real(c_double), intent(in) :: amps(*)
+    end function
+
+    integer(c_int) function sparc_hip_batch_info(batch, ngroups, nlaunch_blocks) &
+         bind(C, name="sparc_hip_batch_info")
+      import :: c_int, c_ptr, c_long
+      type(c_ptr), value :: batch
+      integer(c_int), intent(out) :: ngroups
+      integer(c_long), intent(out) :: nlaunch_blocks
+    end function
+
+    integer(c_int) function sparc_hip_batch_destroy(batch) bind(C, name="sparc_hip_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
     integer(c_int) function sparc_hip_version() bind(C, name="sparc_hip_version")
       import :: c_int
     end function

@@ -28,3 +28,15 @@ def test_fortran_types_match_the_c_header(tmp_path):
         f"-Wl,-rpath,{lib_dir}", f"-L{FLANG_RT}", "-lflang_rt.runtime", "-lm", "-lstdc++")
     out = run(os.path.join(d, "check"))
     assert out.stdout.startswith("OK"), out.stdout
+
+
+def test_fortran_module_binds_every_entry_point():
+    """Each C entry point of the header has a bind(C) interface in the
+    Fortran module (sparc_hip_strerror is covered by sparc_error_message)."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "sparc_hip.h")).read()
+    f90 = open(os.path.join(ROOT, "sparc2_amd", "fortran", "sparc_hip.f90")).read()
+    declared = set(re.findall(r"\b(sparc_hip_\w+)\s*\(", hdr))
+    bound = set(re.findall(r'bind\(C,\s*name="(sparc_hip_\w+)"\)', f90))
+    assert bound <= declared, bound - declared
+    assert declared - bound <= {"sparc_hip_strerror"}, declared - bound

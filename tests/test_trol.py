@@ -1,0 +1,100 @@
+"""Control-file reader: the line rules and order-dependent switches of
+READ_SPARC_PARAMETERS (initialize.f90:1385-1924), on synthetic files and,
+where the reference tree is present, on its own sparc_parameters."""
+import os
+
+import pytest
+
+from sparc2_amd.trol import ParameterError, list_read, read_parameters, to_run_config
+
+REF = "/root/reference/sparc_parameters"
+
+
+def parse(text):
+    return read_parameters(text, is_text=True)
+
+
+def test_list_directed_items():
+    assert list_read("128", "i") == [128]
+    assert list_read(" 200.0d8 ! cm", "r") == [200.0e8]
+    assert list_read("15, 9.2103, 2, 1.0, 0.1", "irrrr") == [15, 9.2103, 2.0, 1.0, 0.1]
+    assert list_read("15 9.2 2 1 .1", "irrrr") == [15, 9.2, 2.0, 1.0, 0.1]
+    assert list_read("4*1.5", "rrrr") == [1.5] * 4
+    assert list_read("1,,3", "iii", [7, 8, 9]) == [1, 8, 3]      # null keeps the value
+    assert list_read("'my run' x", "c") == ["my run"]
+    assert list_read("5 / 6", "i") == [5]
+    with pytest.raises(ParameterError):
+        list_read("12.5", "i")
+    with pytest.raises(ParameterError):
+        list_read("15, 9.2", "irrrr")                               # record too short
+
+
+def test_line_rules():
+    p = parse("!! a comment with x = 3 inside\n"
+              "nx = 64\n"
+              "  ny = 99\n"          # leading blanks: the name matches nothing
+              "NZ = 12\n"            # names are case-sensitive
+              "nz= 40   \n"
+              "timestep = 2.0\n")
+    assert p["nx"] == 64 and p["ny"] == 0 and p["nz"] == 40
+    assert p["stopped_at"] is None
+
+
+def test_failed_read_ends_the_parse():
+    p = parse("nx = 64\nny = sixty\nnz = 40\n")
+    assert p["nx"] == 64 and p["nz"] == 0 and p["stopped_at"] == 2
+
+
+def test_order_dependent_switches():
+    # PML widths survive only if the switch is on when the line is read
+    p = parse("zpmlt = 10, 9.21, 2, 1, 0.1\nUSE_PML = 1\nzpmlb = 12 9.21 2 1 0.1\n")
+    assert p["nzpmltop"] == 0 and p["nzpmlbot"] == 12 and p["zpmlt"] == [9.21, 2.0, 1.0, 0.1]
+    # USE_HPML = 0 switches USE_PML on, as the reference does
+    assert parse("USE_PML = 0\nUSE_HPML = 0\n")["USE_PML"] is True
+    # PERIODIC is refused once USE_HPML is on, honoured before it
+    assert parse("USE_HPML = 1\nPERIODIC = 1\n")["PERIODIC"] is False
+    assert parse("PERIODIC = 1\nUSE_HPML = 1\n")["PERIODIC"] is True
+    # pulse_* values are read only when PULSE is already on
+    p = parse("pulse_amp = 5.0\npulse_dir = 2\nPULSE = 1\npulse_t0 = 300.0\n")
+    assert p["pulse_amp"] == 0.0 and p["pulse_dir"] == 1 and p["pulse_t0"] == 300.0
+    # t_xy / t_z are raised to the timestep read so far
+    p = parse("timestep = 2.0\nt_xy = 1.0\nt_z = 1.0\ntimestep = 0.5\n")
+    assert p["t_xy"] == 2.0 and p["xyfreq_filtering"] == 4
+    # MAGNETIC and the filter switches
+    p = parse("MAGNETIC = 1\nfilter_mode_xy = 2\nfilter_mode_z = 7\n")
+    assert (p["MAGNETIC"], p["QUIET"], p["g_xy_filt"], p["g_z_filt"]) == (False, False, 3, 0)
+
+
+def test_derived_counts():
+    p = parse("timestep = 0.75\nsolartime = 4.0\nt_xy = 15.0\nt_z = 7.5\n"
+              "fulloutputcad = 0\nchunkoutputcad = 45.0\nwall_time = 2.0\nsimname = run\n"
+              "MAGNETIC = 0\n")
+    assert p["maxtime"] == 19200 and p["xyfreq_filtering"] == 20 and p["zfreq_filtering"] == 10
+    assert p["CHUNKOUT"] and p["chunkoutputsteps"] == 60
+    assert not p["FULLOUT"] and p["fulloutputsteps"] == 1
+    assert p["wall_time"] == 7200.0 and p["simname"] == "run_3D_vq_"
+
+
+def test_run_config_mapping():
+    p = parse("nx = 64\nny = 64\nnz = 128\nxlength = 50.0d8\nylength = 50.0d8\ntimestep = 1.0\n"
+              "MAGNETIC = 0\nPERIODIC = 1\nUSE_PML = 1\nzpmlt = 10 9.2103 2 1 0.1\n"
+              "zpmlb = 10 9.2103 2 1 0.1\nfilter_mode_xy = 1\nfilter_mode_z = 1\nt_xy = 30\n"
+              "t_z = 10\nPULSE = 1\npulse_dir = 3\npulse_t0 = 333.3\npulse_px = 25.0d8\n")
+    c = to_run_config(p)
+    assert (c.nx, c.ny, c.nz, c.magnetic, c.periodic) == (64, 64, 128, 0, True)
+    assert c.npml == (0, 0, 0, 0, 10, 10) and c.xyfreq == 30 and c.zfreq == 10
+    assert c.pulse_px == 25.0e8
+    with pytest.raises(ParameterError):
+        to_run_config(parse("TWOD = 1\n"))
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference tree not present")
+def test_reference_parameter_file():
+    p = read_parameters(REF)
+    assert p["stopped_at"] is None
+    assert (p["nx"], p["ny"], p["nz"]) == (128, 128, 220)
+    assert p["MAGNETIC"] and p["PERIODIC"] and p["USE_PML"] and not p["USE_HPML"]
+    assert p["nzpmltop"] == p["nzpmlbot"] == 10 and p["nxpmltop"] == 0
+    assert (p["xyfreq_filtering"], p["zfreq_filtering"], p["maxtime"]) == (20, 10, 19200)
+    c = to_run_config(p)
+    assert c.mhd and c.npml == (0, 0, 0, 0, 10, 10)
