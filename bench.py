@@ -42,8 +42,8 @@ def n_background(cfg):
     return 5
 
 
-def psi_points(cfg):
-    return sum(int(np.prod(cfg.face_shape(f))) for f in range(6) if cfg.face_active(f))
+def psi_points(cfg, faces=range(6)):
+    return sum(int(np.prod(cfg.face_shape(f))) for f in faces if cfg.face_active(f))
 
 
 def alg_bytes_stage(cfg, w):
@@ -52,6 +52,19 @@ def alg_bytes_stage(cfg, w):
     the 3-D backgrounds, and the same three streams for the CPML psi."""
     N = cfg.npoints
     return w * (N * (3 * cfg.nv + n_background(cfg)) + 3 * cfg.npsi * psi_points(cfg))
+
+
+def alg_bytes_zpass(cfg, w):
+    """MHD z pass: read vz, p, bx, by, vx, vy and box, boy, boz; write the six
+    z derivatives; the z-face psi streams."""
+    return w * (cfg.npoints * (9 + 6) + 3 * cfg.npsi * psi_points(cfg, (4, 5)))
+
+
+def alg_bytes_xy(cfg, w):
+    """MHD xy kernel (z derivatives read from the z pass): the stage streams,
+    the 3-D backgrounds, the six z-derivative arrays, the x/y-face psi streams."""
+    N = cfg.npoints
+    return w * (N * (3 * cfg.nv + n_background(cfg) + 6) + 3 * cfg.npsi * psi_points(cfg, (0, 1, 2, 3)))
 
 
 def alg_bytes_step(cfg, w):
@@ -87,6 +100,95 @@ def cpu_baseline(workload, precision, steps, threads):
                       f"steps (1..{steps}) after one untimed step, set-up excluded"}
 
 
+def mode_precision(args):
+    """FP32 fields + Kahan RK update against FP64 on quiet 512x512x300: the
+    driver loop (steps + filters) on both, relative L2 of every field after
+    500, 1000 and 2000 steps, and both throughputs."""
+    from sparc2_amd import workloads
+    from sparc2_amd.solver import Solver
+    cfg64, bg = workloads.build("quiet_precision", precision="fp64")
+    cfg32 = cfg64.replace(precision="fp32")
+    s64 = Solver(cfg64, bg)
+    s32 = Solver(cfg32, bg, setup=None)
+    names = ["rho", "vx", "vy", "vz", "p"]
+    marks = (500, 1000, 2000)
+    l2, t = {}, {"fp64": 0.0, "fp32": 0.0}
+    n0 = 0
+    for m in marks:
+        for key, s in (("fp64", s64), ("fp32", s32)):
+            t0 = time.perf_counter()
+            s.advance(n0, m - 1)
+            s.sync()
+            t[key] += time.perf_counter() - t0
+        a64, a32 = s64.get_state(), s32.get_state()
+        l2[m] = {f: float(np.linalg.norm(a32[q] - a64[q]) / max(np.linalg.norm(a64[q]), 1e-300))
+                 for q, f in enumerate(names)}
+        log(f"[precision] step {m}: {l2[m]}")
+        n0 = m
+    n = marks[-1]
+    g64 = cfg64.npoints * n / t["fp64"] / 1e9
+    g32 = cfg64.npoints * n / t["fp32"] / 1e9
+    print(json.dumps({
+        "metric": METRIC, "mode": "precision", "value": round(g32, 5), "unit": "GPt-updates/s",
+        "n_gpus": 1, "steps": n, "higher_is_better": True, "dtype": "f32+kahan",
+        "data": "synthetic (polytrope background, pulse source)",
+        "config": {"workload": "quiet_precision", "grid": [cfg64.nx, cfg64.ny, cfg64.nz]},
+        "fp64_gpts": round(g64, 5), "fp32_gpts": round(g32, 5), "speedup": round(g32 / g64, 3),
+        "rel_l2": {str(k): v for k, v in l2.items()},
+        "max_rel_l2": max(max(v.values()) for v in l2.values())}), flush=True)
+
+
+def mode_batch(args):
+    """The batched form: a seeded population of independent runs (quiet and
+    MHD, FP32+Kahan) advanced together; aggregate point-updates per second."""
+    from sparc2_amd.batch import Batch
+    from sparc2_amd.solver import Solver
+    from sparc2_amd.workloads import batch_population
+    t0 = time.perf_counter()
+    pop = batch_population(args.runs or 64)
+    sol = [Solver(c, b) for c, b in pop]
+    log(f"[batch] {len(sol)} runs set up in {time.perf_counter() - t0:.1f} s")
+    b = Batch(sol, use_graph=True)
+    b.advance(args.warmup)
+    for s in sol:
+        s.sync()
+    K = args.steps
+    t0 = time.perf_counter()
+    b.advance(K)
+    for s in sol:
+        s.sync()
+    el = time.perf_counter() - t0
+    pts = sum(c.npoints for c, _ in pop)
+    groups, blocks = b.info()
+    n_mhd = sum(1 for c, _ in pop if c.mhd)
+    b.close()
+    print(json.dumps({
+        "metric": METRIC, "mode": "batch", "value": round(pts * K / el / 1e9, 5),
+        "unit": "GPt-updates/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": round(el / K * 1e3, 4), "higher_is_better": True, "dtype": "f32+kahan",
+        "data": "synthetic (seeded population: polytrope / flux-tube backgrounds)",
+        "config": {"workload": "batch_population", "runs": len(pop), "mhd_runs": n_mhd,
+                   "points": pts, "groups": groups, "launch_blocks": blocks, "graph": True}}),
+          flush=True)
+
+
+def mode_serve(args):
+    """The served form: a seeded Poisson stream of requests through the
+    continuous-batching server; runs/h, throughput and latency percentiles."""
+    from sparc2_amd.serve import Server, poisson_stream
+    reqs = poisson_stream(args.runs or 8)
+    r = Server().run(reqs, verbose=True)
+    print(json.dumps({
+        "metric": METRIC, "mode": "serve", "value": round(r["runs_per_h"], 2), "unit": "runs/h",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f32+kahan",
+        "data": "synthetic (seeded Poisson stream, rate 4/s)",
+        "config": {"workload": "poisson_stream", "requests": len(reqs)},
+        "gpts": round(r["gpts"], 5), "gpts_busy": round(r["gpts_busy"], 5),
+        "wall_s": round(r["wall_s"], 3), "latency_p50_s": round(r["latency_p50"], 3),
+        "latency_p95_s": round(r["latency_p95"], 3), "latency_p99_s": round(r["latency_p99"], 3)}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,11 +200,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", default="0,0,0")
+    ap.add_argument("--mode", default="run", choices=["run", "precision", "batch", "serve"],
+                    help="run: the headline single-run line (default); precision: FP32+Kahan "
+                         "vs FP64 relative L2 on quiet 512x512x300; batch: 64 seeded runs in "
+                         "one batch; serve: a Poisson request stream")
+    ap.add_argument("--runs", type=int, default=None,
+                    help="batch: number of runs (64); serve: number of requests (8)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the whole domain lives on one device; "
-              "multi-GPU sharding is not built in this round", file=sys.stderr)
+              "multi-GPU sharding is not built", file=sys.stderr)
         sys.exit(2)
+    if args.mode != "run":
+        {"precision": mode_precision, "batch": mode_batch, "serve": mode_serve}[args.mode](args)
+        return
 
     from sparc2_amd import workloads
     from sparc2_amd.solver import Solver
@@ -136,15 +247,31 @@ def main():
     s.advance(W + K, W + K + P - 1)
     s.sync()
     times = {}
-    for kind, name in ((0, "stage1"), (1, "stage2_4"), (2, "stage5"), (3, "filter")):
+    for kind, name in ((0, "stage1"), (1, "stage2_4"), (2, "stage5"), (3, "filter"), (4, "zpass")):
         n, ms = s.kernel_time(kind)
         times[name] = (n, ms / n if n else None)
     s.timing(False)
     n_mid, avg_mid = times["stage2_4"]
     bytes_stage = alg_bytes_stage(cfg, w)
-    achieved = bytes_stage / (avg_mid * 1e-3) / 1e9
+    zpass = times["zpass"][0] > 0
+    # the dominant kernel: the stage kernel of RK stages 2-4 (for MHD the xy
+    # kernel, whose inputs include the z pass's six derivative arrays)
+    bytes_kernel = alg_bytes_xy(cfg, w) if zpass else bytes_stage
+    achieved = bytes_kernel / (avg_mid * 1e-3) / 1e9
     log(f"[bench] stage kernel avg {avg_mid:.4f} ms over {n_mid} launches, "
-        f"{bytes_stage / 1e9:.3f} GB algorithmic -> {achieved:.1f} GB/s; other kernels {times}")
+        f"{bytes_kernel / 1e9:.3f} GB algorithmic -> {achieved:.1f} GB/s; other kernels {times}")
+    per_kernel = {"k_stage": {"avg_ms": round(avg_mid, 5), "alg_bytes": int(bytes_kernel),
+                              "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+    stage_ms = avg_mid
+    if zpass:
+        zb = alg_bytes_zpass(cfg, w)
+        zms = times["zpass"][1]
+        per_kernel["k_zpass"] = {"avg_ms": round(zms, 5), "alg_bytes": int(zb),
+                                 "frac": round(zb / (zms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        stage_ms = avg_mid + zms
+    # one whole RK stage against the unsplit minimum of its traffic
+    stage_frac = bytes_stage / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    log(f"[bench] RK stage {stage_ms:.4f} ms, {bytes_stage / 1e9:.3f} GB minimum -> {stage_frac:.4f} of peak")
     step_bytes = alg_bytes_step(cfg, w)
     tile_used, nblk = s.tile()
     s.close()
@@ -182,6 +309,8 @@ def main():
             "step_hbm_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "tile": list(tile_used) + [nblk],
             "kernel_avg_ms": {k: (round(v[1], 5) if v[1] else None) for k, v in times.items()},
+            "rk_stage": {"ms": round(stage_ms, 5), "alg_bytes_min": int(bytes_stage),
+                         "frac": round(stage_frac, 4)},
         },
         "roofline": {
             "bound": "hbm",
@@ -190,7 +319,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "k_stage (RK stages 2-4)",
+            "kernel": "k_stage (RK stages 2-4)" + (", MHD xy kernel after k_zpass" if zpass else ""),
+            "per_kernel": per_kernel,
         },
         "cpu_baseline": cpu,
     }

@@ -132,7 +132,8 @@ int sparc_hip_sync(sparc_ctx* ctx);
 /* Kernel timing with HIP events on the context's stream: enable != 0 starts
  * recording every stage launch; sparc_hip_kernel_time returns the number of
  * timed launches and their total milliseconds for kind 0 = RK stage 1,
- * 1 = RK stages 2-4, 2 = RK stage 5 (with the final update), 3 = filter passes. */
+ * 1 = RK stages 2-4, 2 = RK stage 5 (with the final update), 3 = filter passes,
+ * 4 = MHD z passes (the z derivatives each MHD stage launch reads). */
 int sparc_hip_timing(sparc_ctx* ctx, int enable);
 int sparc_hip_kernel_time(sparc_ctx* ctx, int kind, long* count, double* total_ms);
 

@@ -15,7 +15,12 @@ workloads are generated here, deterministically:
   B_r = r R'/R B_z (divergence free for any R(z)).  Gas pressure and density are
   reduced by horizontal magnetohydrostatic balance, p = p_ext - B^2/(8 pi),
   floored at ``p_floor`` * p_ext where the field would exceed the external
-  pressure, with the temperature (sound speed) of the external column.
+  pressure, with the temperature (sound speed) of the external column.  The
+  floor (default 0.5) bounds the density contrast across the tube wall in
+  the low-beta layer above the photosphere: there the tube is not in
+  magnetohydrostatic balance (a 3 kG field cannot be confined by the gas
+  pressure), and with a floor of 0.05 the linearised equations around it
+  grow within a few hundred steps.
 
 Arrays are returned with numpy shape (nz, ny, nx), C order, which is the
 reference's Fortran (nx, ny, nz) memory layout.
@@ -65,7 +70,7 @@ def quiet_background(cfg: RunConfig, zaxis_mm, g=G_SUN):
 
 
 def flux_tube(cfg: RunConfig, zaxis_mm, B0=3000.0, R0=4.0e8, H_R=1.0e8, center=None,
-              p_floor=0.05, g=G_SUN):
+              p_floor=0.5, g=G_SUN):
     rho_e, p_e, c_e = polytrope_column(zaxis_mm, g=g)
     x, y = horizontal_axes(cfg)
     xc, yc = center if center is not None else (0.5 * cfg.xlength, 0.5 * cfg.ylength)

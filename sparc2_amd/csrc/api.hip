@@ -19,7 +19,15 @@
 
 namespace sparc {
 template <typename T>
-hipError_t launch_stage(bool mhd, bool bg3d, int tx, int ty, const StageArgs<T>& A, hipStream_t s);
+hipError_t launch_stage(bool mhd, bool bg3d, bool zp, int tx, int ty, const StageArgs<T>& A,
+                        hipStream_t s);
+template <typename T>
+hipError_t launch_zpass(const StageArgs<T>& A, hipStream_t s);
+template <typename T>
+hipError_t launch_zpass_batch(const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
+                              hipStream_t s);
+long zpass_items(int nx, int ny, int nchunk);
+int zpass_batch_blocks(int ncu);
 bool stage_tile_supported(int tx, int ty);
 template <typename T>
 hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
@@ -29,9 +37,9 @@ hipError_t launch_deriv(int axis, const double* in, double* out, int nx, int ny,
 hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, hipStream_t st);
 hipError_t launch_zero(void* p, long n, bool f32, hipStream_t st);
 template <typename T>
-hipError_t launch_stage_batch(bool mhd, bool bg3d, int tx, int ty, const StageArgs<T>* runs,
+hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int tx, int ty, const StageArgs<T>* runs,
                               const int* item_start, int nrun, int nblk, hipStream_t s);
-int stage_batch_blocks(bool mhd, int tx, int ty, bool f32, int ncu);
+int stage_batch_blocks(bool mhd, bool zp, int tx, int ty, int ncu);
 hipError_t launch_diag_reduce_batch(const void* jobs, int njobs, hipStream_t st);
 hipError_t launch_tick(int* c, int reset, hipStream_t st);
 hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st);
@@ -91,6 +99,10 @@ struct sparc_ctx {
   int ra = 0, rk0 = 1, rk1 = 2;
   void* comp[2] = {nullptr, nullptr};
   int rc = 0;
+  // MHD: the z pass writes the six z derivatives here (6 slabs); zp is off
+  // only for the fused single-kernel MHD stage (SPARC_MHD_FUSED=1)
+  bool zp = false;
+  void* dzbuf = nullptr;
   void* bg[B_COUNT] = {};
   void *g = nullptr, *stretch = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
   double* stretch_d = nullptr;  // FP64 copy for the standalone operators
@@ -99,13 +111,14 @@ struct sparc_ctx {
   double* diag = nullptr;
   double dimc = 1.0, dimrho = 1.0;
   int tx = 64, ty = 8, kc = 8, ntx = 1, nty = 1, nchunk = 1;
+  int zkc = 8, znchunk = 1;  // z-pass chunking
   bool bg_set = false, state_set = false;
   long steps_done = 0;
   double* op_buf = nullptr;  // 2*N doubles for the standalone operators
   bool timing = false;
   std::vector<TimeRec> trecs;
-  long tcount[4] = {0, 0, 0, 0};
-  double tms[4] = {0, 0, 0, 0};
+  long tcount[5] = {0, 0, 0, 0, 0};
+  double tms[5] = {0, 0, 0, 0, 0};
 };
 
 static long round_up(long v, long m) { return (v + m - 1) / m * m; }
@@ -156,6 +169,7 @@ static void free_all(sparc_ctx* c) {
   };
   for (auto& p : c->pool) F(p);
   for (auto& p : c->comp) F(p);
+  F(c->dzbuf);
   for (auto& p : c->bg) F(p);
   F(c->g);
   F(c->stretch);
@@ -180,21 +194,39 @@ static void free_all(sparc_ctx* c) {
   if (c->st) (void)hipStreamDestroy(c->st);
 }
 
-// Tile choice: the x extent covers a 64-wide wavefront row when the grid is
-// wide enough; the z chunk is sized so the launch has >= ~2048 blocks
-// (8 per CU on 256 CUs) without dropping below 4 planes per block.
+// Tile choice.  Shapes the stage kernel is built for, in order of preference
+// per variant (measured on MI355X: quiet FP64 512x512x300 runs fastest at
+// 32x8 -- three 256-thread blocks per CU -- quiet FP32 at 32x16, the MHD xy
+// kernel at 16x16 -- three blocks per CU, where 512-thread tiles fit one).
+// The first shape that fits the grid (one-sided closures need >= 6 rows in
+// the last tile of a non-periodic axis) is taken.  The z chunk is sized so
+// the launch has >= ~2048 blocks (8 per CU on 256 CUs) without dropping below
+// 4 planes per block.
+static bool tile_fits(const sparc_ctx* c, int tx, int ty) {
+  if (!stage_tile_supported(tx, ty)) return false;
+  if (c->cfg.periodic) return true;
+  const int rx = c->nx % tx, ry = c->ny % ty;
+  return ty >= 6 && !(rx > 0 && rx < 6) && !(ry > 0 && ry < 6);
+}
+
 static void choose_tile(sparc_ctx* c) {
   const sparc_cfg& q = c->cfg;
   int tx = q.tile[0], ty = q.tile[1];
   if (!(tx > 0 && ty > 0 && stage_tile_supported(tx, ty))) {
-    // MHD keeps 256-thread blocks: its register queue needs the 512-entry
-    // register file a single wave per SIMD gets.
-    if (c->nx >= 64 && c->nx % 64 == 0 && (q.periodic || !c->mhd)) {
-      tx = 64;
-      ty = c->mhd ? 4 : 8;
-    } else {
-      tx = 32;
-      ty = 8;
+    static const int quiet64[4][2] = {{32, 8}, {16, 16}, {32, 16}, {64, 4}};
+    static const int quiet32[4][2] = {{32, 16}, {32, 8}, {16, 16}, {64, 4}};
+    static const int mhd[4][2] = {{16, 16}, {32, 8}, {32, 16}, {64, 4}};
+    const int(*pref)[2] = c->mhd ? mhd : (c->f32 ? quiet32 : quiet64);
+    tx = 32;
+    ty = 8;
+    for (int n = 0; n < 4; ++n) {
+      // narrow grids: a 16-wide tile wastes less of each block
+      if (c->nx <= 16 && pref[n][0] > 16) continue;
+      if (tile_fits(c, pref[n][0], pref[n][1])) {
+        tx = pref[n][0];
+        ty = pref[n][1];
+        break;
+      }
     }
   }
   c->tx = tx;
@@ -210,18 +242,20 @@ static void choose_tile(sparc_ctx* c) {
   }
   c->kc = kc;
   c->nchunk = (c->nz + kc - 1) / kc;
+  // z pass: about 512 blocks of 256 columns (two per CU), at least 32
+  // planes deep so the 10-plane queue warm-up stays a small overhead
+  const long ncb = ((long)c->nx * c->ny + 255) / 256;
+  c->zkc = (int)std::min<long>(c->nz, std::max<long>(32, (c->nz * ncb + 511) / 512));
+  c->znchunk = (c->nz + c->zkc - 1) / c->zkc;
 }
 
 static int check_tile(sparc_ctx* c) {
   if (!stage_tile_supported(c->tx, c->ty))
     return fail(SPARC_E_INVALID, "unsupported stage tile");
-  if (!c->cfg.periodic) {
-    // one-sided closures need rows 0..10 and n-11..n-1 inside one LDS tile
-    if (c->ty < 6) return fail(SPARC_E_INVALID, "non-periodic runs need tile_y >= 6");
-    int rx = c->nx % c->tx, ry = c->ny % c->ty;
-    if ((rx > 0 && rx < 6) || (ry > 0 && ry < 6))
-      return fail(SPARC_E_INVALID, "non-periodic runs need the last tile to hold >= 6 cells");
-  }
+  // one-sided closures need rows 0..10 and n-11..n-1 inside one LDS tile
+  if (!tile_fits(c, c->tx, c->ty))
+    return fail(SPARC_E_INVALID,
+                "non-periodic runs need tile_y >= 6 and the last tile to hold >= 6 cells");
   return SPARC_OK;
 }
 
@@ -277,6 +311,10 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
   c->esz = c->f32 ? 4 : 8;
   c->mhd = q.option == 3;
   c->bg3d = c->mhd ? true : (q.bg3d != 0);
+  {
+    const char* fused = getenv("SPARC_MHD_FUSED");
+    c->zp = c->mhd && !(fused && fused[0] == '1');
+  }
   c->nv = c->mhd ? 8 : 5;
   c->npsi = c->mhd ? 6 : 2;
   c->nx = q.nx;
@@ -311,6 +349,8 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
   if (c->f32)
     for (int p = 0; p < 2; ++p)
       if ((rc = dalloc(&c->comp[p], sbytes))) return bail(rc);
+  if (c->zp)
+    if ((rc = dalloc(&c->dzbuf, c->esz * 6 * (size_t)c->fs))) return bail(rc);
   const long bgn = c->bg3d ? c->fs : q.nz;
   const int nbg = c->mhd ? B_COUNT : B_P0 + 1;
   for (int s = 0; s < nbg; ++s)
@@ -489,6 +529,8 @@ static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
   A.nty = c->nty;
   A.nblk_xy = c->ntx * c->nty;
   A.nchunk = c->nchunk;
+  A.zkc = c->zkc;
+  A.znchunk = c->znchunk;
   A.periodic = c->cfg.periodic ? 1 : 0;
   A.pulse_dir = c->cfg.pulse_dir;
   A.dt = (T)c->cfg.deltat;
@@ -538,6 +580,7 @@ static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
     A.comp[f] = c->f32 ? (T*)c->comp[c->rc] + (size_t)f * c->fs : nullptr;
     A.a[f] = (T*)c->pool[c->ra] + (size_t)f * c->fs;
   }
+  for (int f = 0; f < 6; ++f) A.dz[f] = c->dzbuf ? (T*)c->dzbuf + (size_t)f * c->fs : nullptr;
 }
 
 // Stage s of LDDRK-5: u_1 = a; stage s reads u_s and writes
@@ -563,8 +606,14 @@ static int do_step(sparc_ctx* c, double amp) {
   for (int s = 1; s <= 5; ++s) {
     set_stage<T>(c, A, s);
     TimeRec r;
+    if (c->zp) {
+      timed_begin(c, 4, &r);
+      hipError_t e = launch_zpass<T>(A, c->st);
+      timed_end(c, &r);
+      if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("z pass launch: ") + hipGetErrorString(e));
+    }
     timed_begin(c, A.first ? 0 : (A.last ? 2 : 1), &r);
-    hipError_t e = launch_stage<T>(c->mhd, c->bg3d, c->tx, c->ty, A, c->st);
+    hipError_t e = launch_stage<T>(c->mhd, c->bg3d, c->zp, c->tx, c->ty, A, c->st);
     timed_end(c, &r);
     if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("stage launch: ") + hipGetErrorString(e));
   }
@@ -708,7 +757,7 @@ int sparc_hip_timing(sparc_ctx* c, int enable) {
 }
 
 int sparc_hip_kernel_time(sparc_ctx* c, int kind, long* count, double* total_ms) {
-  if (!c || kind < 0 || kind > 3 || !count || !total_ms) return fail(SPARC_E_INVALID, "bad argument");
+  if (!c || kind < 0 || kind > 4 || !count || !total_ms) return fail(SPARC_E_INVALID, "bad argument");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   for (auto& r : c->trecs) {
@@ -745,11 +794,12 @@ int sparc_hip_tile(sparc_ctx* c, int tile[3], long* nblocks) {
 // ===========================================================================
 namespace {
 struct BatchGroup {
-  bool mhd = false, bg3d = false;
-  int tx = 0, ty = 0, nblk = 0;
+  bool mhd = false, bg3d = false, zp = false;
+  int tx = 0, ty = 0, nblk = 0, znblk = 0;
   std::vector<int> members;
-  void* args[5] = {};        // StageArgs<T>[members] per stage (device)
-  int* item_start = nullptr;  // prefix of work items (device)
+  void* args[5] = {};         // StageArgs<T>[members] per stage (device)
+  int* item_start = nullptr;   // prefix of stage work items (device)
+  int* zitem_start = nullptr;  // prefix of z-pass work items (device)
 };
 }  // namespace
 
@@ -774,6 +824,7 @@ static void batch_free(sparc_batch* b) {
     for (auto& p : g.args)
       if (p) (void)hipFree(p);
     if (g.item_start) (void)hipFree(g.item_start);
+    if (g.zitem_start) (void)hipFree(g.zitem_start);
   }
   for (auto& p : b->amp)
     if (p) (void)hipFree(p);
@@ -805,8 +856,12 @@ template <typename T>
 static int batch_enqueue_step(sparc_batch* b) {
   for (int s = 1; s <= 5; ++s)
     for (auto& g : b->groups) {
-      hipError_t e = launch_stage_batch<T>(g.mhd, g.bg3d, g.tx, g.ty, (const StageArgs<T>*)g.args[s - 1],
-                                           g.item_start, (int)g.members.size(), g.nblk, b->st);
+      const StageArgs<T>* args = (const StageArgs<T>*)g.args[s - 1];
+      const int nm = (int)g.members.size();
+      hipError_t e = hipSuccess;
+      if (g.zp) e = launch_zpass_batch<T>(args, g.zitem_start, nm, g.znblk, b->st);
+      if (e == hipSuccess)
+        e = launch_stage_batch<T>(g.mhd, g.bg3d, g.zp, g.tx, g.ty, args, g.item_start, nm, g.nblk, b->st);
       if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch stage launch: ") + hipGetErrorString(e));
     }
   if (launch_diag_reduce_batch(b->jobs, (int)b->runs.size(), b->st) != hipSuccess ||
@@ -883,12 +938,13 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
     sparc_ctx* c = b->runs[r];
     BatchGroup* g = nullptr;
     for (auto& h : b->groups)
-      if (h.mhd == c->mhd && h.bg3d == c->bg3d && h.tx == c->tx && h.ty == c->ty) g = &h;
+      if (h.mhd == c->mhd && h.bg3d == c->bg3d && h.zp == c->zp && h.tx == c->tx && h.ty == c->ty) g = &h;
     if (!g) {
       b->groups.emplace_back();
       g = &b->groups.back();
       g->mhd = c->mhd;
       g->bg3d = c->bg3d;
+      g->zp = c->zp;
       g->tx = c->tx;
       g->ty = c->ty;
     }
@@ -907,7 +963,18 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
     if ((rc = dalloc((void**)&g.item_start, sizeof(int) * start.size()))) return bail(rc);
     if (hipMemcpy(g.item_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(SPARC_E_HIP, "item table upload failed"));
-    g.nblk = std::min(stage_batch_blocks(g.mhd, g.tx, g.ty, b->f32, ncu), (start.back() + 7) / 8 * 8);
+    g.nblk = std::min(stage_batch_blocks(g.mhd, g.zp, g.tx, g.ty, ncu), (start.back() + 7) / 8 * 8);
+    if (g.zp) {
+      std::vector<int> zs(g.members.size() + 1, 0);
+      for (size_t q = 0; q < g.members.size(); ++q) {
+        const sparc_ctx* c = b->runs[g.members[q]];
+        zs[q + 1] = zs[q] + (int)zpass_items(c->nx, c->ny, c->znchunk);
+      }
+      if ((rc = dalloc((void**)&g.zitem_start, sizeof(int) * zs.size()))) return bail(rc);
+      if (hipMemcpy(g.zitem_start, zs.data(), sizeof(int) * zs.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(SPARC_E_HIP, "item table upload failed"));
+      g.znblk = std::min(zpass_batch_blocks(ncu), (zs.back() + 7) / 8 * 8);
+    }
   }
   if ((rc = dalloc((void**)&b->tstep, sizeof(int)))) return bail(rc);
   std::vector<DiagJob> jobs(nrun);

@@ -103,6 +103,7 @@ template <typename T> struct StageArgs {
   int nx, ny, nz, kc;
   long fs;
   int ntx, nty, nblk_xy, nchunk;
+  int zkc, znchunk;  // MHD z pass: its own z chunk (deeper: the queue warm-up re-reads 10 planes)
   int periodic, first, last, pulse_dir;
   T beta_next, dt, sxs, sys, src_amp;
   // when amp_tab is set the source amplitude is amp_tab[*tstep] (a device
@@ -113,6 +114,7 @@ template <typename T> struct StageArgs {
   T* a[8];
   T* un[8];
   T* comp[8];  // Kahan compensation (fp32 only), updated in place on the last stage
+  T* dz[6];    // MHD z-pass output: z derivatives in CPML family order
   // backgrounds: 3-D (bg3d) or nz-long z profiles
   const T *rho0, *rhoinv, *c2rho0, *gr0z, *gp0z, *sponge, *p0;
   // MHD backgrounds (always 3-D)
@@ -158,6 +160,18 @@ template <typename T> struct Buf {
       return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(i * 8u), 0, 0));
     } else {
       return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, 0));
+    }
+  }
+  // Load at a byte offset.  kOut lies past the range of every field-slab
+  // view (a slab is a multiple of 1 KiB below 4 GiB), so a load there
+  // returns 0 without a memory access: callers select it instead of
+  // branching around a load.
+  static constexpr uint32_t kOut = 0xFFFFFF00u;
+  __device__ __forceinline__ T ldb(uint32_t boff) const {
+    if constexpr (sizeof(T) == 8) {
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, 0, 0));
+    } else {
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)boff, 0, 0));
     }
   }
   __device__ __forceinline__ void st(uint32_t i, T v) const {

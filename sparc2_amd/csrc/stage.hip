@@ -21,23 +21,27 @@
 
 namespace sparc {
 
-// Occupancy target (waves per SIMD): the MHD register queue (6 fields x 11
-// planes) plus nine LDS-staged fields need about 250 registers, so MHD runs 2
-// waves per SIMD; the quiet variants are held to 3 so memory latency overlaps.
-template <typename T, bool MHD>
+// Occupancy target (waves per SIMD): the fused MHD kernel's register queue (6
+// fields x 11 planes) plus nine LDS-staged fields need about 250 registers,
+// so it runs 2 waves per SIMD; the quiet variants and the MHD xy kernel (z
+// derivatives from the z pass) are held to 3 so memory latency overlaps.
+template <typename T, bool MHD, bool ZP>
 constexpr int stage_waves() {
-  return MHD ? 2 : 3;
+  return (MHD && !ZP) ? 2 : 3;
 }
 
 // One work item: tile `tile` of z chunk `chunk` (logical = chunk*nblk_xy + tile)
-// of the run whose arguments Ap points to.
-template <typename T, bool MHD, bool BG3D, int TX, int TY>
+// of the run whose arguments Ap points to.  ZP (MHD only): the z derivatives,
+// CPML-corrected on the z faces, come from the z pass (Ap->dz) instead of the
+// register queue.
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
 __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
+  static_assert(MHD || !ZP, "the z pass serves the MHD stage only");
   constexpr int NT = TX * TY;
   constexpr int LX = TX + 10, LY = TY + 10;
   constexpr int NR = TY * LX + 10 * TX;  // cross-shaped staging region
   constexpr int NS = (NR + NT - 1) / NT;
-  constexpr int NQ = MHD ? 6 : 2;
+  constexpr int NQ = ZP ? 1 : (MHD ? 6 : 2);
   constexpr int NV = MHD ? 8 : 5;
   // LDS planes: 0 vx (x), 1 vy (y), 2 p (xy); MHD: 3 F3 (xy), 4 F2 (x),
   // 5 F1 (y), 6 by (x), 7 bz (xy), 8 bx (y)
@@ -46,7 +50,9 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   // before the current plane's arithmetic (one barrier per plane).  MHD: one
   // buffer (two would halve the blocks per CU), loads then barrier per plane.
   constexpr int NB = MHD ? 1 : 2;
-  __shared__ T LB[NB][NL][LY * LX];
+  // one spare cell per plane takes the writes of slots past the region
+  constexpr int LP = LY * LX + 1;
+  __shared__ T LB[NB][NL][LP];
   __shared__ double rs[3][NT / 64 + 1];
 
   const int tid = threadIdx.x;
@@ -65,10 +71,15 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   const bool active = (i < nx) && (j < ny);
   const uint32_t ij = active ? (uint32_t)(i + nx * j) : 0u;
 
-  // ---- staging plan: per slot the LDS cell, the global xy offset and
-  // which halo (x or y) the cell belongs to ----
+  // ---- staging plan: per slot the LDS cell and the global xy offset (or
+  // none: outside the domain of a non-periodic axis, or past the region).
+  // Every slot loads and stores unconditionally -- a slot without a source
+  // reads 0 and one past the region writes the spare cell -- so the loads of
+  // a plane issue back to back behind a single wait.  Values staged into a
+  // halo cell that no derivative of that plane reads are harmless. ----
   uint32_t s_off[NS];
-  int s_lds[NS], s_kind[NS];  // kind: 0 none, 1 centre, 2 x halo, 3 y halo
+  bool s_ok[NS];
+  int s_lds[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const int r = tid + s * NT;
@@ -90,41 +101,44 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     } else {
       ok = ok && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
     }
-    const bool xh = col < 5 || col >= TX + 5, yh = row < 5 || row >= TY + 5;
-    s_kind[s] = !ok ? 0 : (xh ? 2 : (yh ? 3 : 1));
+    s_ok[s] = ok;
     s_off[s] = ok ? (uint32_t)(gi + nx * gj) : 0u;
-    s_lds[s] = row * LX + col;
+    s_lds[s] = r < NR ? row * LX + col : LY * LX;
   }
 
   // Array views, rebuilt from the argument pointer where they are used.
   const T sxs = Ap->sxs, sys = Ap->sys;
   auto UF = [&](int f, uint32_t g) -> T { return Buf<T>(Ap->u[f], fs)[g]; };
   auto B3 = [&](const T* p, uint32_t g) -> T { return Buf<T>(p, fs)[g]; };
+  // by byte offset (Buf::kOut reads 0)
+  auto UFb = [&](int f, uint32_t bo) -> T { return Buf<T>(Ap->u[f], fs).ldb(bo); };
+  auto B3b = [&](const T* p, uint32_t bo) -> T { return Buf<T>(p, fs).ldb(bo); };
+  constexpr uint32_t W = sizeof(T);
+  auto slot_bo = [&](int s, uint32_t pk) -> uint32_t {
+    return s_ok[s] ? (pk + s_off[s]) * W : Buf<T>::kOut;
+  };
   auto flux1 = [&](uint32_t g) -> T { return -UF(VZ, g) * B3(Ap->boy, g) + UF(VY, g) * B3(Ap->boz, g); };
   auto flux2 = [&](uint32_t g) -> T { return -UF(VX, g) * B3(Ap->boz, g) + UF(VZ, g) * B3(Ap->box, g); };
 
-  T(*L)[LY * LX] = LB[0];  // the plane buffer in use
+  T(*L)[LP] = LB[0];  // the plane buffer in use
   // quiet: staging values of the next plane, held in registers
   T pre[NS][3];
   auto pre_load = [&](int k) {
     const uint32_t pk = (uint32_t)k * nxy;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      if (s_kind[s] == 0) continue;
-      const uint32_t g = pk + s_off[s];
-      pre[s][0] = UF(VX, g);
-      pre[s][1] = UF(VY, g);
-      pre[s][2] = UF(P, g);
+      const uint32_t bo = slot_bo(s, pk);
+      pre[s][0] = UFb(VX, bo);
+      pre[s][1] = UFb(VY, bo);
+      pre[s][2] = UFb(P, bo);
     }
   };
   auto pre_store = [&]() {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const int kind = s_kind[s];
-      if (kind == 0) continue;
       const int c = s_lds[s];
-      if (kind != 3) L[0][c] = pre[s][0];
-      if (kind != 2) L[1][c] = pre[s][1];
+      L[0][c] = pre[s][0];
+      L[1][c] = pre[s][1];
       L[2][c] = pre[s][2];
     }
   };
@@ -132,24 +146,21 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     const uint32_t pk = (uint32_t)k * nxy;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const int kind = s_kind[s];
-      if (kind == 0) continue;
-      const uint32_t g = pk + s_off[s];
+      const uint32_t bo = slot_bo(s, pk);
       const int c = s_lds[s];
-      const bool wx = kind != 3, wy = kind != 2;  // feeds x / y derivatives
-      const T vx = UF(VX, g), vy = UF(VY, g), p = UF(P, g);
-      if (wx) L[0][c] = vx;
-      if (wy) L[1][c] = vy;
+      const T vx = UFb(VX, bo), vy = UFb(VY, bo), p = UFb(P, bo);
+      L[0][c] = vx;
+      L[1][c] = vy;
       L[2][c] = p;
       if constexpr (MHD) {
-        const T vz = UF(VZ, g), bx = UF(BX, g), by = UF(BY, g), bz = UF(BZ, g);
-        const T b0x = B3(Ap->box, g), b0y = B3(Ap->boy, g), b0z = B3(Ap->boz, g);
-        L[3][c] = -vy * b0x + vx * b0y;           // F3 = -vy box + vx boy
-        if (wx) L[4][c] = -vx * b0z + vz * b0x;   // F2
-        if (wy) L[5][c] = -vz * b0y + vy * b0z;   // F1
-        if (wx) L[6][c] = by;
+        const T vz = UFb(VZ, bo), bx = UFb(BX, bo), by = UFb(BY, bo), bz = UFb(BZ, bo);
+        const T b0x = B3b(Ap->box, bo), b0y = B3b(Ap->boy, bo), b0z = B3b(Ap->boz, bo);
+        L[3][c] = -vy * b0x + vx * b0y;  // F3 = -vy box + vx boy
+        L[4][c] = -vx * b0z + vz * b0x;  // F2
+        L[5][c] = -vz * b0y + vy * b0z;  // F1
+        L[6][c] = by;
         L[7][c] = bz;
-        if (wy) L[8][c] = bx;
+        L[8][c] = bx;
       }
     }
   };
@@ -194,34 +205,33 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   };
 
   // z register queue: slot m holds plane k-5+m of each z-differentiated
-  // field.  Quiet: vz, p.  MHD: vz, p, bx, by, flux1, flux2.
+  // field.  Quiet: vz, p.  MHD: vz, p, bx, by, flux1, flux2.  (Unused with ZP.)
   T q[NQ][11];
   T nq[NQ];  // next plane, loaded one iteration ahead
+  // planes outside 0..nz-1 read as 0: the zero ghosts below the bottom
   auto fetch = [&](int kk, T* dst) {
-    if (active && kk >= 0 && kk < nz) {
-      const uint32_t g = ij + (uint32_t)kk * nxy;
-      dst[0] = UF(VZ, g);
-      dst[1] = UF(P, g);
-      if constexpr (MHD) {
-        dst[2] = UF(BX, g);
-        dst[3] = UF(BY, g);
-        const T vx = UF(VX, g), vy = UF(VY, g), vz = dst[0];
-        dst[4] = -vz * B3(Ap->boy, g) + vy * B3(Ap->boz, g);
-        dst[5] = -vx * B3(Ap->boz, g) + vz * B3(Ap->box, g);
-      }
-    } else {
-#pragma unroll
-      for (int f = 0; f < NQ; ++f) dst[f] = 0;  // zero ghosts below the bottom
+    if constexpr (ZP) return;
+    const uint32_t bo = (active && kk >= 0 && kk < nz) ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
+    dst[0] = UFb(VZ, bo);
+    dst[1] = UFb(P, bo);
+    if constexpr (MHD) {
+      dst[2] = UFb(BX, bo);
+      dst[3] = UFb(BY, bo);
+      const T vx = UFb(VX, bo), vy = UFb(VY, bo), vz = dst[0];
+      dst[4] = -vz * B3b(Ap->boy, bo) + vy * B3b(Ap->boz, bo);
+      dst[5] = -vx * B3b(Ap->boz, bo) + vz * B3b(Ap->box, bo);
     }
   };
+  if constexpr (!ZP) {
 #pragma unroll
-  for (int m = 0; m < 10; ++m) {
-    T t[NQ];
-    fetch(k0 - 5 + m, t);
+    for (int m = 0; m < 10; ++m) {
+      T t[NQ];
+      fetch(k0 - 5 + m, t);
 #pragma unroll
-    for (int f = 0; f < NQ; ++f) q[f][m] = t[f];
+      for (int f = 0; f < NQ; ++f) q[f][m] = t[f];
+    }
+    fetch(k0 + 5, nq);
   }
-  fetch(k0 + 5, nq);
 
   double ke = 0.0, mr = 1.0e300, mp = 1.0e300;
   const T dt = Ap->dt, bnext = Ap->beta_next;
@@ -241,42 +251,43 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     } else {
       stage_plane(k);
     }
-#pragma unroll
-    for (int f = 0; f < NQ; ++f) q[f][10] = nq[f];
-    fetch(k + 6, nq);
-
     // ---- z derivatives (derivatives.f90:481-487,509-518,536-545) ----
     T dz[NQ];
-    if (active) {
-      const T st = ((CPtr<T>)Ap->stretch)[k];
-      if (k < nz - 5) {
+    if constexpr (!ZP) {
 #pragma unroll
-        for (int f = 0; f < NQ; ++f) {
-          T acc = 0;
+      for (int f = 0; f < NQ; ++f) q[f][10] = nq[f];
+      fetch(k + 6, nq);
+      if (active) {
+        const T st = ((CPtr<T>)Ap->stretch)[k];
+        if (k < nz - 5) {
 #pragma unroll
-          for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (q[f][5 + m] - q[f][5 - m]);
-          dz[f] = st * acc;
-        }
-      } else {
-        // top block: mirrored one-sided rows over planes nz-1 .. nz-11
-        const int r = nz - 1 - k;
-        T acc[NQ];
+          for (int f = 0; f < NQ; ++f) {
+            T acc = 0;
 #pragma unroll
-        for (int f = 0; f < NQ; ++f) acc[f] = 0;
-        for (int m = 0; m < 11; ++m) {
-          const uint32_t g = ij + (uint32_t)(nz - 1 - m) * nxy;
-          const T c = Coef<T>::b(r, m);
-          acc[0] += UF(VZ, g) * c;
-          acc[1] += UF(P, g) * c;
-          if constexpr (MHD) {
-            acc[2] += UF(BX, g) * c;
-            acc[3] += UF(BY, g) * c;
-            acc[4] += flux1(g) * c;
-            acc[5] += flux2(g) * c;
+            for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (q[f][5 + m] - q[f][5 - m]);
+            dz[f] = st * acc;
           }
-        }
+        } else {
+          // top block: mirrored one-sided rows over planes nz-1 .. nz-11
+          const int r = nz - 1 - k;
+          T acc[NQ];
 #pragma unroll
-        for (int f = 0; f < NQ; ++f) dz[f] = -st * acc[f];
+          for (int f = 0; f < NQ; ++f) acc[f] = 0;
+          for (int m = 0; m < 11; ++m) {
+            const uint32_t g = ij + (uint32_t)(nz - 1 - m) * nxy;
+            const T c = Coef<T>::b(r, m);
+            acc[0] += UF(VZ, g) * c;
+            acc[1] += UF(P, g) * c;
+            if constexpr (MHD) {
+              acc[2] += UF(BX, g) * c;
+              acc[3] += UF(BY, g) * c;
+              acc[4] += flux1(g) * c;
+              acc[5] += flux2(g) * c;
+            }
+          }
+#pragma unroll
+          for (int f = 0; f < NQ; ++f) dz[f] = -st * acc[f];
+        }
       }
     }
     __syncthreads();
@@ -289,8 +300,10 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       const int cc = (ty + 5) * LX + tx + 5;
       const T vx = L[0][cc], vy = L[1][cc];
       const T rho = UF(RHO, g);
-      const T vz = q[0][5], p = q[1][5];
-      T dvzdz = dz[0], gpz = dz[1];
+      const T vz = ZP ? UF(VZ, g) : q[0][5], p = ZP ? L[2][cc] : q[1][5];
+      // z pass output, in CPML family order: vz, p, bx, by, F2, F1
+      auto DZ = [&](int f) -> T { return Buf<T>(Ap->dz[f], fs)[g]; };
+      T dvzdz = ZP ? DZ(0) : dz[0], gpz = ZP ? DZ(1) : dz[1];
       T dxf3 = 0, dyf3 = 0, dxf2 = 0, dyf1 = 0, dxby = 0, dxbz = 0, dybz = 0, dybx = 0, bz = 0;
       T dzbx = 0, dzby = 0, dzf1 = 0, dzf2 = 0;
       if constexpr (MHD) {
@@ -303,14 +316,23 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         dybz = dy_lds(7);
         dybx = dy_lds(8);
         bz = L[7][cc];
-        dzbx = dz[2];
-        dzby = dz[3];
-        dzf1 = dz[4];
-        dzf2 = dz[5];
+        if constexpr (ZP) {
+          dzbx = DZ(2);
+          dzby = DZ(3);
+          dzf2 = DZ(4);
+          dzf1 = DZ(5);
+        } else {
+          dzbx = dz[2];
+          dzby = dz[3];
+          dzf1 = dz[4];
+          dzf2 = dz[5];
+        }
       }
-      // ---- CPML (RHS.f90:50-93, 424-546) ----
+      // ---- CPML (RHS.f90:50-93, 424-546); with ZP the z faces were done
+      // by the z pass ----
 #pragma unroll
       for (int fz = ZBOT; fz <= ZTOP; ++fz) {
+        if constexpr (ZP) break;
         const auto& F = Ap->face[fz];
         const int kp = k - F.off;
         if (!F.active || kp < 0 || kp >= F.n) continue;
@@ -370,7 +392,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       } else {
         const T red = B3(Ap->red, g);
         const T b0x = B3(Ap->box, g), b0y = B3(Ap->boy, g), b0z = B3(Ap->boz, g);
-        const T bx = q[2][5], by = q[3][5];
+        const T bx = ZP ? L[8][cc] : q[2][5], by = ZP ? L[6][cc] : q[3][5];
         // RHS.f90:548-580
         K[BX] = red * (dyf3 - dzf2);
         K[BY] = red * (dzf1 - dxf3);
@@ -444,10 +466,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     // single buffer: the next plane's staging must wait for these reads
     if constexpr (NB == 1) __syncthreads();
 
+    if constexpr (!ZP) {
 #pragma unroll
-    for (int f = 0; f < NQ; ++f)
+      for (int f = 0; f < NQ; ++f)
 #pragma unroll
-      for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
+        for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
+    }
   }
 
   if (last) {
@@ -480,24 +504,162 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// z pass of the MHD stage: the six z derivatives of ddz (derivatives.f90:
+// 451-564) -- vz, p, bx, by and the fluxes F2 = -vx box... (in CPML family
+// order: vz, p, bx, by, F2, F1) -- with the z-face CPML applied
+// (RHS.f90:424-546), written to Ap->dz for the xy kernel.  One column per
+// thread, NT consecutive columns per block, marching a z chunk with an
+// 11-deep register queue; no LDS and no barriers, so the waves of a block
+// stream independently.  Work item: column block cb of z chunk `chunk`
+// (logical = chunk*ncb + cb).
+// ---------------------------------------------------------------------------
+constexpr int ZP_NT = 256;
+
+template <typename T>
+__device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
+  constexpr int NT = ZP_NT;
+  const int nz = Ap->nz;
+  const uint32_t fs = (uint32_t)Ap->fs;
+  const uint32_t nxy = (uint32_t)Ap->nx * (uint32_t)Ap->ny;
+  const int ncb = (int)((nxy + NT - 1) / NT), kc = Ap->zkc;
+  const int cb = logical % ncb, chunk = logical / ncb;
+  const uint32_t col = (uint32_t)cb * NT + threadIdx.x;
+  const bool active = col < nxy;
+  const uint32_t ij = active ? col : 0u;
+  const int k0 = chunk * kc, k1 = min(k0 + kc, nz);
+  const bool first = Ap->first, last = Ap->last;
+  const T dt = Ap->dt, bnext = Ap->beta_next;
+  constexpr uint32_t W = sizeof(T);
+
+  auto UF = [&](int f, uint32_t g) -> T { return Buf<T>(Ap->u[f], fs)[g]; };
+  auto B3 = [&](const T* p, uint32_t g) -> T { return Buf<T>(p, fs)[g]; };
+  auto UFb = [&](int f, uint32_t bo) -> T { return Buf<T>(Ap->u[f], fs).ldb(bo); };
+  auto B3b = [&](const T* p, uint32_t bo) -> T { return Buf<T>(p, fs).ldb(bo); };
+  // the six fields at plane kk in family order; outside 0..nz-1 they read 0
+  // (the zero ghosts below the bottom)
+  auto fetch = [&](int kk, T* dst) {
+    const uint32_t bo = (active && kk >= 0 && kk < nz) ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
+    const T vx = UFb(VX, bo), vy = UFb(VY, bo), vz = UFb(VZ, bo);
+    const T b0x = B3b(Ap->box, bo), b0y = B3b(Ap->boy, bo), b0z = B3b(Ap->boz, bo);
+    dst[0] = vz;
+    dst[1] = UFb(P, bo);
+    dst[2] = UFb(BX, bo);
+    dst[3] = UFb(BY, bo);
+    dst[4] = -vx * b0z + vz * b0x;  // F2
+    dst[5] = -vz * b0y + vy * b0z;  // F1
+  };
+  auto fields_at = [&](uint32_t g, T* dst) {
+    const T vx = UF(VX, g), vy = UF(VY, g), vz = UF(VZ, g);
+    const T b0x = B3(Ap->box, g), b0y = B3(Ap->boy, g), b0z = B3(Ap->boz, g);
+    dst[0] = vz;
+    dst[1] = UF(P, g);
+    dst[2] = UF(BX, g);
+    dst[3] = UF(BY, g);
+    dst[4] = -vx * b0z + vz * b0x;
+    dst[5] = -vz * b0y + vy * b0z;
+  };
+
+  T q[6][11], nq[6];
+#pragma unroll
+  for (int m = 0; m < 10; ++m) {
+    T t[6];
+    fetch(k0 - 5 + m, t);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) q[f][m] = t[f];
+  }
+  fetch(k0 + 5, nq);
+
+  for (int k = k0; k < k1; ++k) {
+    SPARC_REFRESH(Ap);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) q[f][10] = nq[f];
+    fetch(k + 6, nq);
+    const T st = ((CPtr<T>)Ap->stretch)[k];
+    T d[6];
+    if (k < nz - 5) {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        T acc = 0;
+#pragma unroll
+        for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (q[f][5 + m] - q[f][5 - m]);
+        d[f] = st * acc;
+      }
+    } else {
+      // top block: mirrored one-sided rows over planes nz-1 .. nz-11
+      const int r = nz - 1 - k;
+      T acc[6] = {0, 0, 0, 0, 0, 0};
+      for (int m = 0; m < 11; ++m) {
+        T v[6];
+        fields_at(ij + (uint32_t)(nz - 1 - m) * nxy, v);
+        const T c = Coef<T>::b(r, m);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) acc[f] += v[f] * c;
+      }
+#pragma unroll
+      for (int f = 0; f < 6; ++f) d[f] = -st * acc[f];
+    }
+    if (active) {
+      const uint32_t g = ij + (uint32_t)k * nxy;
+#pragma unroll
+      for (int fz = ZBOT; fz <= ZTOP; ++fz) {
+        const auto& F = Ap->face[fz];
+        const int kp = k - F.off;
+        if (!F.active || kp < 0 || kp >= F.n) continue;
+        const uint32_t fp = ij + nxy * (uint32_t)kp;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) pml_family(F, fp, f, d[f], first, last, bnext, dt);
+      }
+#pragma unroll
+      for (int f = 0; f < 6; ++f) Buf<T>(Ap->dz[f], fs).st(g, d[f]);
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+#pragma unroll
+      for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(ZP_NT, 2) void k_zpass(const StageArgs<T> A) {
+  const int nblk = gridDim.x, b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
+  const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
+  zpass_item<T>((CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr(), logical);
+}
+
+template <typename T>
+__global__ __launch_bounds__(ZP_NT, 2) void k_zpass_batch(const StageArgs<T>* __restrict__ runs,
+                                                           const int* __restrict__ item_start, int nrun) {
+  const int total = item_start[nrun];
+  const int xcd = blockIdx.x & 7, m = blockIdx.x >> 3, nb8 = gridDim.x >> 3;
+  const int lo = (int)((long)total * xcd / 8), hi = (int)((long)total * (xcd + 1) / 8);
+  for (int item = lo + m; item < hi; item += nb8) {
+    int r = 0;
+    while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
+    zpass_item<T>((CArgs<T>)(runs + r), item - item_start[r]);
+  }
+}
+
 // Single run: one block per work item.  The hardware hands consecutive block
 // ids to the 8 XCDs in turn; remap so each XCD gets a contiguous run of tiles
 // and the halos the tiles share stay in that XCD's L2.
-template <typename T, bool MHD, bool BG3D, int TX, int TY>
-__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage(const StageArgs<T> A) {
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
+__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, ZP>())) void k_stage(const StageArgs<T> A) {
   const int nblk = gridDim.x, b = blockIdx.x;
   const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
   const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
   // A sits at offset 0 of the kernel-argument segment
-  stage_item<T, MHD, BG3D, TX, TY>((CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr(), logical);
+  stage_item<T, MHD, BG3D, TX, TY, ZP>((CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr(), logical);
 }
 
 // Many runs in one persistent launch: the work items of all runs form one
 // list (item_start[r] = first item of run r, item_start[nrun] = total).  Each
 // XCD takes a contiguous eighth of the list and its blocks stride through it.
 // The grid size must be a multiple of 8.
-template <typename T, bool MHD, bool BG3D, int TX, int TY>
-__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage_batch(
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
+__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, ZP>())) void k_stage_batch(
     const StageArgs<T>* __restrict__ runs, const int* __restrict__ item_start, int nrun) {
   const int total = item_start[nrun];
   const int xcd = blockIdx.x & 7, m = blockIdx.x >> 3, nb8 = gridDim.x >> 3;
@@ -506,7 +668,7 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage_batch
     int r = 0;
     while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
     // no kernel writes the argument array while the batch runs
-    stage_item<T, MHD, BG3D, TX, TY>((CArgs<T>)(runs + r), item - item_start[r]);
+    stage_item<T, MHD, BG3D, TX, TY, ZP>((CArgs<T>)(runs + r), item - item_start[r]);
     __syncthreads();
   }
 }
@@ -514,77 +676,107 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD>())) void k_stage_batch
 // ---------------------------------------------------------------------------
 // Host launchers (explicit instantiations over the supported tile shapes).
 // ---------------------------------------------------------------------------
-template <typename T, bool MHD, bool BG3D, int TX, int TY>
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
 static hipError_t launch_one(const StageArgs<T>& A, hipStream_t s) {
   const int nblk = A.nblk_xy * A.nchunk;
-  hipLaunchKernelGGL((k_stage<T, MHD, BG3D, TX, TY>), dim3(nblk), dim3(TX * TY), 0, s, A);
+  hipLaunchKernelGGL((k_stage<T, MHD, BG3D, TX, TY, ZP>), dim3(nblk), dim3(TX * TY), 0, s, A);
   return hipGetLastError();
 }
 
-template <typename T, bool MHD, bool BG3D>
+template <typename T, bool MHD, bool BG3D, bool ZP>
 static hipError_t launch_tile(int tx, int ty, const StageArgs<T>& A, hipStream_t s) {
-  if (tx == 64 && ty == 4) return launch_one<T, MHD, BG3D, 64, 4>(A, s);
-  if (tx == 64 && ty == 8) return launch_one<T, MHD, BG3D, 64, 8>(A, s);
-  if (tx == 32 && ty == 8) return launch_one<T, MHD, BG3D, 32, 8>(A, s);
-  if (tx == 32 && ty == 16) return launch_one<T, MHD, BG3D, 32, 16>(A, s);
-  if (tx == 16 && ty == 16) return launch_one<T, MHD, BG3D, 16, 16>(A, s);
+  if (tx == 64 && ty == 4) return launch_one<T, MHD, BG3D, 64, 4, ZP>(A, s);
+  if (tx == 64 && ty == 8) return launch_one<T, MHD, BG3D, 64, 8, ZP>(A, s);
+  if (tx == 32 && ty == 8) return launch_one<T, MHD, BG3D, 32, 8, ZP>(A, s);
+  if (tx == 32 && ty == 16) return launch_one<T, MHD, BG3D, 32, 16, ZP>(A, s);
+  if (tx == 16 && ty == 16) return launch_one<T, MHD, BG3D, 16, 16, ZP>(A, s);
   return hipErrorInvalidValue;
 }
 
-template <typename T, bool MHD, bool BG3D, int TX, int TY>
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
 static hipError_t launch_batch_one(const StageArgs<T>* runs, const int* item_start, int nrun,
                                    int nblk, hipStream_t s) {
-  hipLaunchKernelGGL((k_stage_batch<T, MHD, BG3D, TX, TY>), dim3(nblk), dim3(TX * TY), 0, s, runs,
-                     item_start, nrun);
+  hipLaunchKernelGGL((k_stage_batch<T, MHD, BG3D, TX, TY, ZP>), dim3(nblk), dim3(TX * TY), 0, s,
+                     runs, item_start, nrun);
   return hipGetLastError();
 }
 
-template <typename T, bool MHD, bool BG3D>
+template <typename T, bool MHD, bool BG3D, bool ZP>
 static hipError_t launch_batch_tile(int tx, int ty, const StageArgs<T>* runs, const int* item_start,
                                     int nrun, int nblk, hipStream_t s) {
-  if (tx == 64 && ty == 4) return launch_batch_one<T, MHD, BG3D, 64, 4>(runs, item_start, nrun, nblk, s);
-  if (tx == 64 && ty == 8) return launch_batch_one<T, MHD, BG3D, 64, 8>(runs, item_start, nrun, nblk, s);
-  if (tx == 32 && ty == 8) return launch_batch_one<T, MHD, BG3D, 32, 8>(runs, item_start, nrun, nblk, s);
-  if (tx == 32 && ty == 16) return launch_batch_one<T, MHD, BG3D, 32, 16>(runs, item_start, nrun, nblk, s);
-  if (tx == 16 && ty == 16) return launch_batch_one<T, MHD, BG3D, 16, 16>(runs, item_start, nrun, nblk, s);
+  if (tx == 64 && ty == 4) return launch_batch_one<T, MHD, BG3D, 64, 4, ZP>(runs, item_start, nrun, nblk, s);
+  if (tx == 64 && ty == 8) return launch_batch_one<T, MHD, BG3D, 64, 8, ZP>(runs, item_start, nrun, nblk, s);
+  if (tx == 32 && ty == 8) return launch_batch_one<T, MHD, BG3D, 32, 8, ZP>(runs, item_start, nrun, nblk, s);
+  if (tx == 32 && ty == 16) return launch_batch_one<T, MHD, BG3D, 32, 16, ZP>(runs, item_start, nrun, nblk, s);
+  if (tx == 16 && ty == 16) return launch_batch_one<T, MHD, BG3D, 16, 16, ZP>(runs, item_start, nrun, nblk, s);
   return hipErrorInvalidValue;
 }
 
 template <typename T>
-hipError_t launch_stage_batch(bool mhd, bool bg3d, int tx, int ty, const StageArgs<T>* runs,
+hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int tx, int ty, const StageArgs<T>* runs,
                               const int* item_start, int nrun, int nblk, hipStream_t s) {
-  if (mhd) return launch_batch_tile<T, true, true>(tx, ty, runs, item_start, nrun, nblk, s);
-  if (bg3d) return launch_batch_tile<T, false, true>(tx, ty, runs, item_start, nrun, nblk, s);
-  return launch_batch_tile<T, false, false>(tx, ty, runs, item_start, nrun, nblk, s);
+  if (mhd && zp) return launch_batch_tile<T, true, true, true>(tx, ty, runs, item_start, nrun, nblk, s);
+  if (mhd) return launch_batch_tile<T, true, true, false>(tx, ty, runs, item_start, nrun, nblk, s);
+  if (bg3d) return launch_batch_tile<T, false, true, false>(tx, ty, runs, item_start, nrun, nblk, s);
+  return launch_batch_tile<T, false, false, false>(tx, ty, runs, item_start, nrun, nblk, s);
 }
 
-template hipError_t launch_stage_batch<double>(bool, bool, int, int, const StageArgs<double>*,
+template hipError_t launch_stage_batch<double>(bool, bool, bool, int, int, const StageArgs<double>*,
                                                const int*, int, int, hipStream_t);
-template hipError_t launch_stage_batch<float>(bool, bool, int, int, const StageArgs<float>*,
+template hipError_t launch_stage_batch<float>(bool, bool, bool, int, int, const StageArgs<float>*,
                                               const int*, int, int, hipStream_t);
 
 // Occupancy-derived persistent grid for a batch launch (blocks resident at once).
-int stage_batch_blocks(bool mhd, int tx, int ty, bool f32, int ncu) {
-  const int waves = mhd ? 2 : 3;
-  (void)f32;
+int stage_batch_blocks(bool mhd, bool zp, int tx, int ty, int ncu) {
+  const int waves = (mhd && !zp) ? 2 : 3;
   const int per_cu = (4 * waves * 64) / (tx * ty);
   const int n = ncu * (per_cu > 0 ? per_cu : 1);
   return (n + 7) / 8 * 8;
 }
 
 template <typename T>
-hipError_t launch_stage(bool mhd, bool bg3d, int tx, int ty, const StageArgs<T>& A, hipStream_t s) {
-  if (mhd) return launch_tile<T, true, true>(tx, ty, A, s);
-  if (bg3d) return launch_tile<T, false, true>(tx, ty, A, s);
-  return launch_tile<T, false, false>(tx, ty, A, s);
+hipError_t launch_stage(bool mhd, bool bg3d, bool zp, int tx, int ty, const StageArgs<T>& A,
+                        hipStream_t s) {
+  if (mhd && zp) return launch_tile<T, true, true, true>(tx, ty, A, s);
+  if (mhd) return launch_tile<T, true, true, false>(tx, ty, A, s);
+  if (bg3d) return launch_tile<T, false, true, false>(tx, ty, A, s);
+  return launch_tile<T, false, false, false>(tx, ty, A, s);
 }
 
-template hipError_t launch_stage<double>(bool, bool, int, int, const StageArgs<double>&, hipStream_t);
-template hipError_t launch_stage<float>(bool, bool, int, int, const StageArgs<float>&, hipStream_t);
+template hipError_t launch_stage<double>(bool, bool, bool, int, int, const StageArgs<double>&, hipStream_t);
+template hipError_t launch_stage<float>(bool, bool, bool, int, int, const StageArgs<float>&, hipStream_t);
 
 bool stage_tile_supported(int tx, int ty) {
   return (tx == 64 && (ty == 4 || ty == 8)) || (tx == 32 && (ty == 8 || ty == 16)) ||
          (tx == 16 && ty == 16);
 }
+
+// z pass: work items per run (column blocks x z chunks), single-run and
+// batched launches, and the batch grid (two 256-thread blocks per CU).
+long zpass_items(int nx, int ny, int nchunk) {
+  const long nxy = (long)nx * ny;
+  return (nxy + ZP_NT - 1) / ZP_NT * nchunk;
+}
+
+template <typename T>
+hipError_t launch_zpass(const StageArgs<T>& A, hipStream_t s) {
+  const long nblk = zpass_items(A.nx, A.ny, A.znchunk);
+  hipLaunchKernelGGL((k_zpass<T>), dim3((unsigned)nblk), dim3(ZP_NT), 0, s, A);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_zpass_batch(const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
+                              hipStream_t s) {
+  hipLaunchKernelGGL((k_zpass_batch<T>), dim3(nblk), dim3(ZP_NT), 0, s, runs, item_start, nrun);
+  return hipGetLastError();
+}
+
+int zpass_batch_blocks(int ncu) { return ncu * 2; }
+
+template hipError_t launch_zpass<double>(const StageArgs<double>&, hipStream_t);
+template hipError_t launch_zpass<float>(const StageArgs<float>&, hipStream_t);
+template hipError_t launch_zpass_batch<double>(const StageArgs<double>*, const int*, int, int, hipStream_t);
+template hipError_t launch_zpass_batch<float>(const StageArgs<float>*, const int*, int, int, hipStream_t);
 
 }  // namespace sparc

@@ -110,7 +110,7 @@ def poisson_stream(n: int, rate: float = 4.0, seed: int = 7, precision: str = "f
             nx=nx, ny=ny, nz=nz, xlength=L, ylength=L, timestep=1.0, magnetic=2 if mhd else 0,
             periodic=not mhd, use_pml=True, use_hpml=mhd,
             npml=(10, 10, 10, 10, 10, 10) if mhd else (0, 0, 0, 0, 10, 10),
-            pmlc=((9.2103, 2.0, 1.0, 0.1),) * 6, t_xy=30.0, t_z=10.0,
+            pmlc=((9.2103, 2.0, 8.0, 0.1),) * 6, t_xy=30.0, t_z=10.0,
             pulse_t0=1000.0 / 3.0, pulse_t1=450.0, pulse_st=150.0,
             pulse_px=float(px), pulse_py=float(py), pulse_pz=-0.15e8,
             pulse_sr=1.0e8, pulse_sz=1.0e8, pulse_amp=1.0e4, maxva=1.0e7, precision=precision)

@@ -4,12 +4,14 @@
   periodic, z in [-10, +0.5] Mm, polytrope background, vertical pulse on vz at
   the domain centre and z = -0.15 Mm (sigma 1 Mm), 3 mHz carrier
   (t0 = 333.33 s), 11-point stencils, z filter every 10 s, xy filter every
-  30 s, 10-cell CPML top and bottom, dt = 1 s, 500 steps.
+  30 s, 10-cell CPML top and bottom (kappa 8), dt = 1 s, 500 steps.
 * ``sunspot_mhd`` -- plain sunspot MHD run: 256^3, 100 Mm, z in [-20, +1] Mm,
   flux tube (3 kG on axis, Gaussian radius 4 Mm at z = 0), Alfven speed capped
   at 100 km/s, CPML on all six faces (N = 15, RC = ln(1e4) = 9.2103, n = 2,
   kappa = 1, sponge 0.1), pulse 10 Mm off axis, dt = 0.5 s, 2000 steps.
 * ``quiet_precision`` -- quiet_acoustic's physics at 512 x 512 x 300.
+* ``batch_population`` -- a seeded mix of quiet and MHD runs for the batched
+  form.
 
 Reference facts these follow: the pulse is sin x Gaussian (driver.f90:140-145;
 there is no Ricker option), xy stencils are 11-point like z
@@ -22,7 +24,11 @@ from __future__ import annotations
 from .background import make_background
 from .config import RunConfig
 
-PML_Z = (9.2103, 2.0, 1.0, 0.1)
+# CPML line "RC, n, kappa, sponge".  The quiet runs use kappa = 8 (the value of
+# the reference's sample control file): with kappa = 1 the z-CPML of the
+# quiet 512 x 512 x 300 run lets a slowly growing mode through after ~1500
+# steps, with kappa = 8 it stays bounded past 3000.
+PML_Z = (9.2103, 2.0, 8.0, 0.1)
 
 
 def quiet_acoustic(nx=64, ny=64, nz=128, precision="fp64", **kw):
@@ -39,8 +45,8 @@ def quiet_acoustic(nx=64, ny=64, nz=128, precision="fp64", **kw):
     return cfg, (-10.0, 0.5)
 
 
-def sunspot_mhd(nx=256, ny=256, nz=256, precision="fp64", **kw):
-    pml = (9.2103, 2.0, 1.0, 0.1)
+def sunspot_mhd(nx=256, ny=256, nz=256, precision="fp64", kappa=1.0, **kw):
+    pml = (9.2103, 2.0, kappa, 0.1)
     cfg = RunConfig(
         nx=nx, ny=ny, nz=nz, xlength=100.0e8, ylength=100.0e8, timestep=0.5,
         solartime=1000.0 / 3600.0, magnetic=2, periodic=False, use_pml=True, use_hpml=True,
@@ -68,3 +74,37 @@ def build(name: str, **kw):
     else:
         bg = make_background(cfg, zb, zt)
     return cfg, bg
+
+
+def batch_population(n=64, seed=2024, precision="fp32"):
+    """The batched workload: n independent runs drawn with a seeded generator
+    -- nx, ny from {48, 64, 96, 128, 192}, nz uniform in 96..256, B0 from
+    {0, 0.5, ..., 3.0} kG (0 gives the quiet kernel, otherwise the flux-tube
+    MHD run on sunspot_mhd's 100 Mm box with 10-cell CPML, kappa 8, on all
+    faces), the
+    source uniform in the central half of the box, dt from the CFL estimate.
+    Returns [(cfg, raw background)]."""
+    import numpy as np
+
+    from .serve import cfl_timestep
+    rng = np.random.default_rng(seed)
+    sizes = (48, 64, 96, 128, 192)
+    out = []
+    for _ in range(n):
+        nx, ny = (int(sizes[rng.integers(len(sizes))]) for _ in range(2))
+        nz = int(rng.integers(96, 257))
+        B0 = 500.0 * int(rng.integers(0, 7))
+        mhd = B0 > 0.0
+        L = 100.0e8 if mhd else 50.0e8
+        px, py = rng.uniform(0.25 * L, 0.75 * L, size=2)
+        if mhd:
+            cfg, zr = sunspot_mhd(nx=nx, ny=ny, nz=nz, precision=precision, kappa=8.0,
+                                  npml=(10, 10, 10, 10, 10, 10))
+        else:
+            cfg, zr = quiet_acoustic(nx=nx, ny=ny, nz=nz, precision=precision)
+        cfg.pulse_px, cfg.pulse_py = float(px), float(py)
+        cfg.__post_init__()
+        bg = make_background(cfg, *zr, B0=B0, R0=4.0e8) if mhd else make_background(cfg, *zr)
+        cfg.timestep = cfl_timestep(cfg, bg)
+        out.append((cfg, bg))
+    return out
