@@ -79,6 +79,25 @@ def alg_bytes_step(cfg, w):
     return b
 
 
+def profiled_traffic(workload, precision, tile):
+    """HBM bytes per launch of the stage kernel, from the committed rocprofv3
+    summary profiles/<workload>_<precision>.md (FETCH_SIZE and WRITE_SIZE
+    scaled by a 1 GiB copy), when it was taken with the same tile shape."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{precision}.md")
+    if not os.path.exists(path):
+        return None, None
+    text = open(path).read()
+    try:
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        summ = json.loads(block)
+    except (IndexError, ValueError):
+        return None, None
+    if list(summ.get("tile") or []) != list(tile[:2]):
+        return None, None
+    k = summ["kernels"].get("k_stage")
+    return (k["traffic_bytes_per_launch"] if k else None), os.path.relpath(path, ROOT)
+
+
 def cpu_baseline(workload, precision, steps, threads):
     """The oracle (C restatement, OpenMP) on the same workload; only the
     time-step loop is timed."""
@@ -189,6 +208,30 @@ def mode_serve(args):
           flush=True)
 
 
+def mode_profile(args):
+    """What rocprofv3 runs: one device copy of 1 GiB (its FETCH_SIZE /
+    WRITE_SIZE reading calibrates the counters against a known byte count),
+    then --steps driver steps of the workload.  No timing of its own."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    torch.cuda.empty_cache()
+    from sparc2_amd import workloads
+    from sparc2_amd.solver import Solver
+    cfg, bg = workloads.build(args.workload, precision=args.precision)
+    tile = tuple(int(v) for v in args.tile.split(","))
+    s = Solver(cfg, bg, tile=tile)
+    s.advance(1, args.steps)
+    s.sync()
+    print(json.dumps({"mode": "profile", "workload": args.workload, "precision": args.precision,
+                      "steps": args.steps, "tile": list(s.tile()[0]), "copy_bytes": 1 << 30}),
+          flush=True)
+    s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,10 +243,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", default="0,0,0")
-    ap.add_argument("--mode", default="run", choices=["run", "precision", "batch", "serve"],
+    ap.add_argument("--mode", default="run", choices=["run", "precision", "batch", "serve", "profile"],
                     help="run: the headline single-run line (default); precision: FP32+Kahan "
                          "vs FP64 relative L2 on quiet 512x512x300; batch: 64 seeded runs in "
-                         "one batch; serve: a Poisson request stream")
+                         "one batch; serve: a Poisson request stream; profile: a 1 GiB copy "
+                         "(the HBM-counter calibration) then --steps driver steps, for rocprofv3")
     ap.add_argument("--runs", type=int, default=None,
                     help="batch: number of runs (64); serve: number of requests (8)")
     args = ap.parse_args()
@@ -212,7 +256,8 @@ def main():
               "multi-GPU sharding is not built", file=sys.stderr)
         sys.exit(2)
     if args.mode != "run":
-        {"precision": mode_precision, "batch": mode_batch, "serve": mode_serve}[args.mode](args)
+        {"precision": mode_precision, "batch": mode_batch, "serve": mode_serve,
+         "profile": mode_profile}[args.mode](args)
         return
 
     from sparc2_amd import workloads
@@ -274,6 +319,7 @@ def main():
     log(f"[bench] RK stage {stage_ms:.4f} ms, {bytes_stage / 1e9:.3f} GB minimum -> {stage_frac:.4f} of peak")
     step_bytes = alg_bytes_step(cfg, w)
     tile_used, nblk = s.tile()
+    traffic, traffic_src = profiled_traffic(args.workload, args.precision, tile_used)
     s.close()
 
     cpu = None
@@ -318,7 +364,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "k_stage (RK stages 2-4)" + (", MHD xy kernel after k_zpass" if zpass else ""),
             "per_kernel": per_kernel,
         },
