@@ -167,7 +167,10 @@ def mode_batch(args):
     pop = batch_population(args.runs or 64)
     sol = [Solver(c, b) for c, b in pop]
     log(f"[batch] {len(sol)} runs set up in {time.perf_counter() - t0:.1f} s")
-    b = Batch(sol, use_graph=True)
+    t0 = time.perf_counter()
+    b = Batch(sol, use_graph=True, autotune=not args.no_autotune)
+    tiles = sorted({s.tile()[0][:2] for s in sol})
+    log(f"[batch] autotuned in {time.perf_counter() - t0:.1f} s: tile shapes {tiles}")
     b.advance(args.warmup)
     for s in sol:
         s.sync()
@@ -181,13 +184,33 @@ def mode_batch(args):
     groups, blocks = b.info()
     n_mhd = sum(1 for c, _ in pop if c.mhd)
     b.close()
+    plain = None
+    if args.fp64_steps > 0:
+        # the same runs, one at a time, plain FP64 (the reference's own path)
+        for s in sol:
+            s.close()
+        tot = 0.0
+        for c, bgr in pop:
+            s = Solver(c.replace(precision="fp64"), bgr)
+            s.advance(0, 1)
+            s.sync()
+            t1 = time.perf_counter()
+            s.advance(2, 1 + args.fp64_steps)
+            s.sync()
+            tot += time.perf_counter() - t1
+            s.close()
+        plain = pts * args.fp64_steps / tot / 1e9
+        log(f"[batch] plain FP64, runs one at a time: {plain:.4f} GPt/s")
     print(json.dumps({
         "metric": METRIC, "mode": "batch", "value": round(pts * K / el / 1e9, 5),
         "unit": "GPt-updates/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
         "ms_per_step": round(el / K * 1e3, 4), "higher_is_better": True, "dtype": "f32+kahan",
         "data": "synthetic (seeded population: polytrope / flux-tube backgrounds)",
         "config": {"workload": "batch_population", "runs": len(pop), "mhd_runs": n_mhd,
-                   "points": pts, "groups": groups, "launch_blocks": blocks, "graph": True}}),
+                   "points": pts, "groups": groups, "launch_blocks": blocks, "graph": True,
+                   "autotuned_tiles": [list(t) for t in tiles]},
+        "plain_fp64_gpts": round(plain, 5) if plain else None,
+        "speedup_vs_plain_fp64": round(pts * K / el / 1e9 / plain, 3) if plain else None}),
           flush=True)
 
 
@@ -248,6 +271,9 @@ def main():
                          "vs FP64 relative L2 on quiet 512x512x300; batch: 64 seeded runs in "
                          "one batch; serve: a Poisson request stream; profile: a 1 GiB copy "
                          "(the HBM-counter calibration) then --steps driver steps, for rocprofv3")
+    ap.add_argument("--no-autotune", action="store_true", help="batch: keep the default tiles")
+    ap.add_argument("--fp64-steps", type=int, default=0,
+                    help="batch: also time this many steps of every run alone in plain FP64")
     ap.add_argument("--runs", type=int, default=None,
                     help="batch: number of runs (64); serve: number of requests (8)")
     args = ap.parse_args()

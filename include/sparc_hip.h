@@ -140,6 +140,17 @@ int sparc_hip_kernel_time(sparc_ctx* ctx, int kind, long* count, double* total_m
 /* Tile (x, y, z-chunk) the stage kernel uses and its launch size. */
 int sparc_hip_tile(sparc_ctx* ctx, int tile[3], long* nblocks);
 
+/* Tile autotuner (no counterpart in the reference; it serves the batched and
+ * served forms, whose runs differ in grid size).  Times `reps` launches of a
+ * middle RK stage for every supported tile shape that fits the grid, at the
+ * automatic z chunk and twice it, and keeps the fastest: tile_out = (tx, ty,
+ * kc), ms_out = its time per stage.  A middle stage writes only buffers the
+ * next step overwrites, so the run's state is unchanged.  Call between steps
+ * after set_background and put_state.  sparc_hip_set_tile applies a known
+ * choice (kc <= 0: automatic); it fails for a shape that does not fit. */
+int sparc_hip_autotune(sparc_ctx* ctx, int reps, int tile_out[3], double* ms_out);
+int sparc_hip_set_tile(sparc_ctx* ctx, const int tile[3]);
+
 /* Set-up helper: the Alfven-limiter decimal search of initialize.f90:962-986
  * (the search temperature T; the host then sets reduction = T/(T + red)).
  * red = |B0|^2/(rho0 cq^2), b2 = |B0|^2, rho0: nx*ny*nz nondimensional arrays;

@@ -20,10 +20,14 @@ from .solver import Solver
 
 
 class Batch:
-    def __init__(self, solvers: Sequence[Solver], use_graph: bool = True):
+    def __init__(self, solvers: Sequence[Solver], use_graph: bool = True, autotune: bool = False):
         self.solvers: List[Solver] = list(solvers)
         if not self.solvers:
             raise ValueError("empty batch")
+        if autotune:
+            # one timing sweep per distinct grid; runs alike reuse it
+            for s in self.solvers:
+                s.autotune()
         n = len(self.solvers)
         hs = (C.c_void_p * n)(*[s.h.value for s in self.solvers])
         h = C.c_void_p()

@@ -27,7 +27,7 @@ template <typename T>
 hipError_t launch_zpass_batch(const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
                               hipStream_t s);
 long zpass_items(int nx, int ny, int nchunk);
-int zpass_batch_blocks(int ncu);
+int zpass_batch_blocks(bool f32, int ncu);
 bool stage_tile_supported(int tx, int ty);
 template <typename T>
 hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
@@ -39,7 +39,7 @@ hipError_t launch_zero(void* p, long n, bool f32, hipStream_t st);
 template <typename T>
 hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int tx, int ty, const StageArgs<T>* runs,
                               const int* item_start, int nrun, int nblk, hipStream_t s);
-int stage_batch_blocks(bool mhd, bool zp, int tx, int ty, int ncu);
+int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int tx, int ty, int ncu);
 hipError_t launch_diag_reduce_batch(const void* jobs, int njobs, hipStream_t st);
 hipError_t launch_tick(int* c, int reset, hipStream_t st);
 hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st);
@@ -108,6 +108,7 @@ struct sparc_ctx {
   double* stretch_d = nullptr;  // FP64 copy for the standalone operators
   FaceDev face[6];
   double* partial = nullptr;
+  long partial_cap = 0;  // blocks the partial buffer holds
   double* diag = nullptr;
   double dimc = 1.0, dimrho = 1.0;
   int tx = 64, ty = 8, kc = 8, ntx = 1, nty = 1, nchunk = 1;
@@ -195,9 +196,10 @@ static void free_all(sparc_ctx* c) {
 }
 
 // Tile choice.  Shapes the stage kernel is built for, in order of preference
-// per variant (measured on MI355X: quiet FP64 512x512x300 runs fastest at
-// 32x8 -- three 256-thread blocks per CU -- quiet FP32 at 32x16, the MHD xy
-// kernel at 16x16 -- three blocks per CU, where 512-thread tiles fit one).
+// per variant (measured on MI355X, 512x512x300 quiet and 256^3 MHD: quiet
+// FP64 and FP32 run fastest at 32x8 -- three 256-thread blocks per CU -- the
+// MHD xy kernel at 16x16 in FP64 (a 512-thread tile fits only one block per
+// CU) and at 32x16 in FP32 (two fit)).
 // The first shape that fits the grid (one-sided closures need >= 6 rows in
 // the last tile of a non-periodic axis) is taken.  The z chunk is sized so
 // the launch has >= ~2048 blocks (8 per CU on 256 CUs) without dropping below
@@ -209,14 +211,17 @@ static bool tile_fits(const sparc_ctx* c, int tx, int ty) {
   return ty >= 6 && !(rx > 0 && rx < 6) && !(ry > 0 && ry < 6);
 }
 
+static void set_geometry(sparc_ctx* c, int tx, int ty, int kc);
+
 static void choose_tile(sparc_ctx* c) {
   const sparc_cfg& q = c->cfg;
   int tx = q.tile[0], ty = q.tile[1];
   if (!(tx > 0 && ty > 0 && stage_tile_supported(tx, ty))) {
     static const int quiet64[4][2] = {{32, 8}, {16, 16}, {32, 16}, {64, 4}};
-    static const int quiet32[4][2] = {{32, 16}, {32, 8}, {16, 16}, {64, 4}};
-    static const int mhd[4][2] = {{16, 16}, {32, 8}, {32, 16}, {64, 4}};
-    const int(*pref)[2] = c->mhd ? mhd : (c->f32 ? quiet32 : quiet64);
+    static const int quiet32[4][2] = {{32, 8}, {64, 4}, {16, 16}, {32, 16}};
+    static const int mhd64[4][2] = {{16, 16}, {32, 8}, {32, 16}, {64, 4}};
+    static const int mhd32[4][2] = {{32, 16}, {32, 8}, {16, 16}, {64, 4}};
+    const int(*pref)[2] = c->mhd ? (c->f32 ? mhd32 : mhd64) : (c->f32 ? quiet32 : quiet64);
     tx = 32;
     ty = 8;
     for (int n = 0; n < 4; ++n) {
@@ -229,12 +234,16 @@ static void choose_tile(sparc_ctx* c) {
       }
     }
   }
+  set_geometry(c, tx, ty, q.tile[2]);
+}
+
+// Tile shape, z chunk (kc <= 0: automatic) and the z pass's chunking.
+static void set_geometry(sparc_ctx* c, int tx, int ty, int kc) {
   c->tx = tx;
   c->ty = ty;
   c->ntx = (c->nx + tx - 1) / tx;
   c->nty = (c->ny + ty - 1) / ty;
   long tiles = (long)c->ntx * c->nty;
-  int kc = q.tile[2];
   if (kc <= 0) {
     long want = 2048;
     kc = (int)std::max<long>(4, (c->nz * tiles + want - 1) / want);
@@ -363,6 +372,7 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
   if ((rc = dalloc((void**)&c->stretch_d, sizeof(double) * q.nz))) return bail(rc);
   const long nblk = (long)c->ntx * c->nty * c->nchunk;
   if ((rc = dalloc((void**)&c->partial, sizeof(double) * 3 * nblk))) return bail(rc);
+  c->partial_cap = nblk;
   if ((rc = dalloc((void**)&c->diag, sizeof(double) * 3))) return bail(rc);
   for (int f = 0; f < 6; ++f) {
     FaceDev& F = c->face[f];
@@ -774,6 +784,93 @@ int sparc_hip_kernel_time(sparc_ctx* c, int kind, long* count, double* total_ms)
   return SPARC_OK;
 }
 
+// Re-tile a context between steps (the partial-sum buffer grows if needed).
+static int retile(sparc_ctx* c, int tx, int ty, int kc) {
+  if (!tile_fits(c, tx, ty)) return fail(SPARC_E_INVALID, "tile shape unsupported or does not fit the grid");
+  set_geometry(c, tx, ty, kc);
+  const long nblk = (long)c->ntx * c->nty * c->nchunk;
+  if (nblk > c->partial_cap) {
+    if (c->partial) (void)hipFree(c->partial);
+    c->partial = nullptr;
+    c->partial_cap = 0;
+    int rc = dalloc((void**)&c->partial, sizeof(double) * 3 * nblk);
+    if (rc) return rc;
+    c->partial_cap = nblk;
+  }
+  return SPARC_OK;
+}
+
+int sparc_hip_set_tile(sparc_ctx* c, const int tile[3]) {
+  if (!c || !tile) return fail(SPARC_E_INVALID, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return retile(c, tile[0], tile[1], tile[2]);
+}
+
+// Average time of one middle RK stage (stage 2: reads the first stage buffer,
+// writes the second, the CPML stage values and, for MHD, the z-pass slabs --
+// all overwritten by the next step, so the run's state is untouched).
+template <typename T>
+static int time_middle_stage(sparc_ctx* c, int reps, double* ms) {
+  StageArgs<T> A;
+  fill_common<T>(c, A);
+  set_stage<T>(c, A, 2);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  hipError_t e = hipSuccess;
+  for (int q = 0; q <= reps && e == hipSuccess; ++q) {
+    if (q == 1) e = hipEventRecord(e0, c->st);
+    if (e == hipSuccess && c->zp) e = launch_zpass<T>(A, c->st);
+    if (e == hipSuccess) e = launch_stage<T>(c->mhd, c->bg3d, c->zp, c->tx, c->ty, A, c->st);
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, c->st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("autotune launch: ") + hipGetErrorString(e));
+  *ms = t / reps;
+  return SPARC_OK;
+}
+
+int sparc_hip_autotune(sparc_ctx* c, int reps, int tile_out[3], double* ms_out) {
+  if (!c || !tile_out || !ms_out || reps < 1) return fail(SPARC_E_INVALID, "bad argument");
+  if (!c->bg_set || !c->state_set) return fail(SPARC_E_STATE, "set_background and put_state must precede autotune");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  static const int shapes[5][2] = {{16, 16}, {32, 8}, {32, 16}, {64, 4}, {64, 8}};
+  int best[3] = {c->tx, c->ty, c->kc};
+  double best_ms = 1e30;
+  for (const auto& sh : shapes) {
+    if (!tile_fits(c, sh[0], sh[1])) continue;
+    for (int mult = 1; mult <= 2; ++mult) {
+      int rc = retile(c, sh[0], sh[1], 0);
+      if (rc) return rc;
+      const int kc = std::min(c->nz, c->kc * mult);
+      if (mult == 2 && kc == c->kc) continue;
+      if ((rc = retile(c, sh[0], sh[1], kc))) return rc;
+      double ms = 0.0;
+      rc = c->f32 ? time_middle_stage<float>(c, reps, &ms) : time_middle_stage<double>(c, reps, &ms);
+      if (rc) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best[0] = sh[0];
+        best[1] = sh[1];
+        best[2] = kc;
+      }
+    }
+  }
+  int rc = retile(c, best[0], best[1], best[2]);
+  if (rc) return rc;
+  tile_out[0] = best[0];
+  tile_out[1] = best[1];
+  tile_out[2] = best[2];
+  *ms_out = best_ms;
+  return SPARC_OK;
+}
+
 int sparc_hip_tile(sparc_ctx* c, int tile[3], long* nblocks) {
   if (!c || !tile || !nblocks) return fail(SPARC_E_INVALID, "null argument");
   tile[0] = c->tx;
@@ -963,7 +1060,8 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
     if ((rc = dalloc((void**)&g.item_start, sizeof(int) * start.size()))) return bail(rc);
     if (hipMemcpy(g.item_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(SPARC_E_HIP, "item table upload failed"));
-    g.nblk = std::min(stage_batch_blocks(g.mhd, g.zp, g.tx, g.ty, ncu), (start.back() + 7) / 8 * 8);
+    g.nblk = std::min(stage_batch_blocks(b->f32, g.mhd, g.bg3d, g.zp, g.tx, g.ty, ncu),
+                      (start.back() + 7) / 8 * 8);
     if (g.zp) {
       std::vector<int> zs(g.members.size() + 1, 0);
       for (size_t q = 0; q < g.members.size(); ++q) {
@@ -973,7 +1071,7 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
       if ((rc = dalloc((void**)&g.zitem_start, sizeof(int) * zs.size()))) return bail(rc);
       if (hipMemcpy(g.zitem_start, zs.data(), sizeof(int) * zs.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(SPARC_E_HIP, "item table upload failed"));
-      g.znblk = std::min(zpass_batch_blocks(ncu), (zs.back() + 7) / 8 * 8);
+      g.znblk = std::min(zpass_batch_blocks(b->f32, ncu), (zs.back() + 7) / 8 * 8);
     }
   }
   if ((rc = dalloc((void**)&b->tstep, sizeof(int)))) return bail(rc);

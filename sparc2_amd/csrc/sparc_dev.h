@@ -184,25 +184,34 @@ template <typename T> struct Buf {
   }
 };
 
-// PML update of one derivative family at one point (RHS.f90:52-55): the psi
-// RHS comes from the raw derivative, then the derivative is replaced by
-// d/kappa + psi.  The next psi stage value (step.f90:34-36) or, on the last
-// stage, the advanced base (step.f90:59-61) is written in place.  FR is a
-// Face<T> in any address space.
-template <typename T, typename FR>
-__device__ __forceinline__ void pml_family(FR& F, uint32_t fp, int q, T& d, bool first, bool last,
-                                           T beta_next, T dt) {
+// CPML update at one point of one face (RHS.f90:52-55 per derivative
+// family): the psi RHS comes from the raw derivative, then the derivative is
+// replaced by d/kappa + psi; the next psi stage value (step.f90:34-36) or, on
+// the last stage, the advanced base (step.f90:59-61) is written in place.
+// v[i] is the derivative of psi family fam[i].  All loads of the face point
+// (a, b, 1/kappa once, then every family's base and stage value) are issued
+// before any store, so they share one memory wait.  FR is a Face<T> in any
+// address space.
+template <typename T, int NF, typename FR>
+__device__ __forceinline__ void pml_face(FR& F, uint32_t fp, T (&v)[NF], const int (&fam)[NF],
+                                         bool first, bool last, T beta_next, T dt) {
   const uint32_t n = (uint32_t)F.fsize;
-  const Buf<T> psi(F.psi + (size_t)q * n, n), pst(F.pst + (size_t)q * n, n);
-  const Buf<T> fa(F.a, n), fb(F.b, n), ki(F.kinv, n);
-  const T base = psi[fp];
-  const T ps = first ? base : pst[fp];
-  const T kn = fa[fp] * d + fb[fp] * ps;
-  d = d * ki[fp] + ps;
-  if (last)
-    psi.st(fp, base + dt * kn);
-  else
-    pst.st(fp, base + beta_next * kn);
+  const T ca = Buf<T>(F.a, n)[fp], cb = Buf<T>(F.b, n)[fp], ck = Buf<T>(F.kinv, n)[fp];
+  T base[NF], ps[NF];
+#pragma unroll
+  for (int i = 0; i < NF; ++i) {
+    base[i] = Buf<T>(F.psi + (size_t)fam[i] * n, n)[fp];
+    ps[i] = first ? base[i] : Buf<T>(F.pst + (size_t)fam[i] * n, n)[fp];
+  }
+#pragma unroll
+  for (int i = 0; i < NF; ++i) {
+    const T kn = ca * v[i] + cb * ps[i];
+    v[i] = v[i] * ck + ps[i];
+    if (last)
+      Buf<T>(F.psi + (size_t)fam[i] * n, n).st(fp, base[i] + dt * kn);
+    else
+      Buf<T>(F.pst + (size_t)fam[i] * n, n).st(fp, base[i] + beta_next * kn);
+  }
 }
 
 __device__ __forceinline__ int wrap(int i, int n) {

@@ -241,13 +241,48 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     else return ((CPtr<T>)p)[k];
   };
 
+  // Quiet FP32: the point values of a plane -- rho, the RK base a, and the 3-D
+  // backgrounds where present -- are loaded one plane ahead too, so every load
+  // a plane needs was issued a whole iteration earlier and the iteration
+  // waits once, at its top.  (In-order completion counting would otherwise
+  // make the point loads wait for the next plane's staging loads as well.)
+  // In FP64 the extra registers would cost a wave per SIMD, so FP64 loads the
+  // point values in the plane itself.
+  constexpr bool PPF = !MHD && sizeof(T) == 4;
+  constexpr int NPB = (PPF && BG3D) ? 6 : 0;  // rho0, rhoinv, sponge, gr0z, c2rho0, gp0z
+  constexpr int NP = PPF ? 1 + NV + NPB : 1;
+  T pc[NP], pn[NP];
+  auto point_load = [&](int kk, T* dst) {
+    if constexpr (PPF) {
+      const uint32_t bo = active ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
+      dst[0] = UFb(RHO, bo);
+#pragma unroll
+      for (int f = 0; f < NV; ++f) dst[1 + f] = Buf<T>(Ap->a[f], fs).ldb(bo);
+      if constexpr (NPB > 0) {
+        dst[1 + NV + 0] = B3b(Ap->rho0, bo);
+        dst[1 + NV + 1] = B3b(Ap->rhoinv, bo);
+        dst[1 + NV + 2] = B3b(Ap->sponge, bo);
+        dst[1 + NV + 3] = B3b(Ap->gr0z, bo);
+        dst[1 + NV + 4] = B3b(Ap->c2rho0, bo);
+        dst[1 + NV + 5] = B3b(Ap->gp0z, bo);
+      }
+    }
+  };
   if constexpr (!MHD) pre_load(k0);
+  if constexpr (PPF) point_load(k0, pn);
   for (int k = k0; k < k1; ++k) {
     SPARC_REFRESH(Ap);
     L = LB[NB == 2 ? (k & 1) : 0];
     if constexpr (!MHD) {
       pre_store();
-      if (k + 1 < k1) pre_load(k + 1);
+      if constexpr (PPF) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) pc[i] = pn[i];
+      }
+      if (k + 1 < k1) {
+        pre_load(k + 1);
+        if constexpr (PPF) point_load(k + 1, pn);
+      }
     } else {
       stage_plane(k);
     }
@@ -299,7 +334,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       T dvxdx = dx_lds(0), dvydy = dy_lds(1), gpx = dx_lds(2), gpy = dy_lds(2);
       const int cc = (ty + 5) * LX + tx + 5;
       const T vx = L[0][cc], vy = L[1][cc];
-      const T rho = UF(RHO, g);
+      const T rho = PPF ? pc[0] : UF(RHO, g);
       const T vz = ZP ? UF(VZ, g) : q[0][5], p = ZP ? L[2][cc] : q[1][5];
       // z pass output, in CPML family order: vz, p, bx, by, F2, F1
       auto DZ = [&](int f) -> T { return Buf<T>(Ap->dz[f], fs)[g]; };
@@ -337,13 +372,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         const int kp = k - F.off;
         if (!F.active || kp < 0 || kp >= F.n) continue;
         const uint32_t fp = ij + nxy * (uint32_t)kp;
-        pml_family(F, fp, 1, gpz, first, last, bnext, dt);
-        pml_family(F, fp, 0, dvzdz, first, last, bnext, dt);
         if constexpr (MHD) {
-          pml_family(F, fp, 4, dzf2, first, last, bnext, dt);
-          pml_family(F, fp, 5, dzf1, first, last, bnext, dt);
-          pml_family(F, fp, 2, dzbx, first, last, bnext, dt);
-          pml_family(F, fp, 3, dzby, first, last, bnext, dt);
+          T v[6] = {gpz, dvzdz, dzf2, dzf1, dzbx, dzby};
+          constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
+          pml_face(F, fp, v, fam, first, last, bnext, dt);
+          gpz = v[0], dvzdz = v[1], dzf2 = v[2], dzf1 = v[3], dzbx = v[4], dzby = v[5];
+        } else {
+          T v[2] = {gpz, dvzdz};
+          constexpr int fam[2] = {1, 0};
+          pml_face(F, fp, v, fam, first, last, bnext, dt);
+          gpz = v[0], dvzdz = v[1];
         }
       }
 #pragma unroll
@@ -352,13 +390,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         const int ip = i - F.off;
         if (!F.active || ip < 0 || ip >= F.n) continue;
         const uint32_t fp = (uint32_t)(ip + F.n * (j + ny * k));
-        pml_family(F, fp, 1, gpx, first, last, bnext, dt);
-        pml_family(F, fp, 0, dvxdx, first, last, bnext, dt);
         if constexpr (MHD) {
-          pml_family(F, fp, 4, dxf3, first, last, bnext, dt);
-          pml_family(F, fp, 5, dxf2, first, last, bnext, dt);
-          pml_family(F, fp, 2, dxby, first, last, bnext, dt);
-          pml_family(F, fp, 3, dxbz, first, last, bnext, dt);
+          T v[6] = {gpx, dvxdx, dxf3, dxf2, dxby, dxbz};
+          constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
+          pml_face(F, fp, v, fam, first, last, bnext, dt);
+          gpx = v[0], dvxdx = v[1], dxf3 = v[2], dxf2 = v[3], dxby = v[4], dxbz = v[5];
+        } else {
+          T v[2] = {gpx, dvxdx};
+          constexpr int fam[2] = {1, 0};
+          pml_face(F, fp, v, fam, first, last, bnext, dt);
+          gpx = v[0], dvxdx = v[1];
         }
       }
 #pragma unroll
@@ -367,28 +408,36 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         const int jp = j - F.off;
         if (!F.active || jp < 0 || jp >= F.n) continue;
         const uint32_t fp = (uint32_t)(i + nx * (jp + F.n * k));
-        pml_family(F, fp, 1, gpy, first, last, bnext, dt);
-        pml_family(F, fp, 0, dvydy, first, last, bnext, dt);
         if constexpr (MHD) {
-          pml_family(F, fp, 4, dyf3, first, last, bnext, dt);
-          pml_family(F, fp, 5, dyf1, first, last, bnext, dt);
-          pml_family(F, fp, 2, dybx, first, last, bnext, dt);
-          pml_family(F, fp, 3, dybz, first, last, bnext, dt);
+          T v[6] = {gpy, dvydy, dyf3, dyf1, dybx, dybz};
+          constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
+          pml_face(F, fp, v, fam, first, last, bnext, dt);
+          gpy = v[0], dvydy = v[1], dyf3 = v[2], dyf1 = v[3], dybx = v[4], dybz = v[5];
+        } else {
+          T v[2] = {gpy, dvydy};
+          constexpr int fam[2] = {1, 0};
+          pml_face(F, fp, v, fam, first, last, bnext, dt);
+          gpy = v[0], dvydy = v[1];
         }
       }
 
       // ---- pointwise RHS ----
-      const T rho0 = BG(Ap->rho0, g, k), rhoinv = BG(Ap->rhoinv, g, k), sp = BG(Ap->sponge, g, k);
+      // background at the point (quiet 3-D backgrounds come from the prefetch)
+      auto BGP = [&](const T* p, int slot) -> T {
+        if constexpr (NPB > 0) return pc[1 + NV + slot];
+        else return BG(p, g, k);
+      };
+      const T rho0 = BGP(Ap->rho0, 0), rhoinv = BGP(Ap->rhoinv, 1), sp = BGP(Ap->sponge, 2);
       const T div = dvxdx + dvydy + dvzdz;
       T K[8];
       if constexpr (!MHD) {
         const T gk = ((CPtr<T>)Ap->g)[k];
         // RHS.f90:95-101,136,152
-        K[RHO] = -BG(Ap->gr0z, g, k) * vz - rho0 * div - sp * rho;
+        K[RHO] = -BGP(Ap->gr0z, 3) * vz - rho0 * div - sp * rho;
         K[VX] = -rhoinv * gpx - sp * vx;
         K[VY] = -rhoinv * gpy - sp * vy;
         K[VZ] = -rhoinv * (gpz + rho * gk) - vz * sp;
-        K[P] = -BG(Ap->c2rho0, g, k) * div - vz * BG(Ap->gp0z, g, k) - sp * p;
+        K[P] = -BGP(Ap->c2rho0, 4) * div - vz * BGP(Ap->gp0z, 5) - sp * p;
       } else {
         const T red = B3(Ap->red, g);
         const T b0x = B3(Ap->box, g), b0y = B3(Ap->boy, g), b0z = B3(Ap->boz, g);
@@ -432,15 +481,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         // next stage value, step.f90:33
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
-          const Buf<T> af(Ap->a[f], fs), uf(Ap->un[f], fs);
-          uf.st(g, af[g] + bnext * K[f]);
+          const Buf<T> uf(Ap->un[f], fs);
+          const T a0 = PPF ? pc[1 + f] : Buf<T>(Ap->a[f], fs)[g];
+          uf.st(g, a0 + bnext * K[f]);
         }
       } else {
         T an[NV];
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
           const Buf<T> af(Ap->a[f], fs);
-          const T a0 = af[g];
+          const T a0 = PPF ? pc[1 + f] : af[g];
           if constexpr (sizeof(T) == 4) {
             // compensated update: a - c carries the bits a + dt K loses
             const Buf<T> cf(Ap->comp[f], fs);
@@ -608,8 +658,8 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
         const int kp = k - F.off;
         if (!F.active || kp < 0 || kp >= F.n) continue;
         const uint32_t fp = ij + nxy * (uint32_t)kp;
-#pragma unroll
-        for (int f = 0; f < 6; ++f) pml_family(F, fp, f, d[f], first, last, bnext, dt);
+        constexpr int fam[6] = {0, 1, 2, 3, 4, 5};
+        pml_face(F, fp, d, fam, first, last, bnext, dt);
       }
 #pragma unroll
       for (int f = 0; f < 6; ++f) Buf<T>(Ap->dz[f], fs).st(g, d[f]);
@@ -726,12 +776,43 @@ template hipError_t launch_stage_batch<double>(bool, bool, bool, int, int, const
 template hipError_t launch_stage_batch<float>(bool, bool, bool, int, int, const StageArgs<float>*,
                                               const int*, int, int, hipStream_t);
 
-// Occupancy-derived persistent grid for a batch launch (blocks resident at once).
-int stage_batch_blocks(bool mhd, bool zp, int tx, int ty, int ncu) {
-  const int waves = (mhd && !zp) ? 2 : 3;
-  const int per_cu = (4 * waves * 64) / (tx * ty);
-  const int n = ncu * (per_cu > 0 ? per_cu : 1);
+// Persistent grid for a batch launch: the blocks of this kernel instance that
+// are resident at once (the runtime's occupancy for its registers and LDS),
+// rounded up to a multiple of 8 (one share per XCD).
+template <typename K>
+static int resident_grid(K kernel, int threads, int ncu) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  const int n = ncu * per_cu;
   return (n + 7) / 8 * 8;
+}
+
+template <typename T, bool MHD, bool BG3D, bool ZP>
+static int batch_blocks_tile(int tx, int ty, int ncu) {
+#define SPARC_OCC(X, Y) \
+  if (tx == X && ty == Y) return resident_grid(k_stage_batch<T, MHD, BG3D, X, Y, ZP>, X * Y, ncu);
+  SPARC_OCC(64, 4)
+  SPARC_OCC(64, 8)
+  SPARC_OCC(32, 8)
+  SPARC_OCC(32, 16)
+  SPARC_OCC(16, 16)
+#undef SPARC_OCC
+  return ncu;
+}
+
+template <typename T>
+static int batch_blocks_t(bool mhd, bool bg3d, bool zp, int tx, int ty, int ncu) {
+  if (mhd && zp) return batch_blocks_tile<T, true, true, true>(tx, ty, ncu);
+  if (mhd) return batch_blocks_tile<T, true, true, false>(tx, ty, ncu);
+  if (bg3d) return batch_blocks_tile<T, false, true, false>(tx, ty, ncu);
+  return batch_blocks_tile<T, false, false, false>(tx, ty, ncu);
+}
+
+int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int tx, int ty, int ncu) {
+  return f32 ? batch_blocks_t<float>(mhd, bg3d, zp, tx, ty, ncu)
+             : batch_blocks_t<double>(mhd, bg3d, zp, tx, ty, ncu);
 }
 
 template <typename T>
@@ -772,7 +853,10 @@ hipError_t launch_zpass_batch(const StageArgs<T>* runs, const int* item_start, i
   return hipGetLastError();
 }
 
-int zpass_batch_blocks(int ncu) { return ncu * 2; }
+int zpass_batch_blocks(bool f32, int ncu) {
+  return f32 ? resident_grid(k_zpass_batch<float>, ZP_NT, ncu)
+             : resident_grid(k_zpass_batch<double>, ZP_NT, ncu);
+}
 
 template hipError_t launch_zpass<double>(const StageArgs<double>&, hipStream_t);
 template hipError_t launch_zpass<float>(const StageArgs<float>&, hipStream_t);

@@ -53,7 +53,9 @@ module sparc_hip
        sparc_hip_advance, sparc_hip_diag, sparc_hip_ddx, sparc_hip_ddy, sparc_hip_ddz, &
        sparc_hip_filtx, sparc_hip_filty, sparc_hip_filtz, sparc_hip_sync, &
        sparc_hip_timing, sparc_hip_kernel_time, sparc_hip_tile, sparc_hip_version, &
-       sparc_error_message
+       sparc_hip_alfven_search, sparc_hip_get_slab, sparc_hip_batch_create, &
+       sparc_hip_batch_advance, sparc_hip_batch_info, sparc_hip_batch_destroy, &
+       sparc_hip_autotune, sparc_hip_set_tile, sparc_error_message
 
   interface
     integer(c_int) function sparc_hip_create(cfg, ctx) bind(C, name="sparc_hip_create")
@@ -234,6 +236,22 @@ module sparc_hip
     integer(c_int) function sparc_hip_batch_destroy(batch) bind(C, name="sparc_hip_batch_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: batch
+    end function
+
+    ! tile autotuner: tile_out = (tx, ty, kc), ms_out = time per middle stage
+    integer(c_int) function sparc_hip_autotune(ctx, reps, tile_out, ms_out) &
+         bind(C, name="sparc_hip_autotune")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: reps
+      integer(c_int), intent(out) :: tile_out(3)
+      real(c_double), intent(out) :: ms_out
+    end function
+
+    integer(c_int) function sparc_hip_set_tile(ctx, tile) bind(C, name="sparc_hip_set_tile")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: tile(3)
     end function
 
     integer(c_int) function sparc_hip_version() bind(C, name="sparc_hip_version")

@@ -26,6 +26,10 @@ from .initialise import (SetupState, initialise_rhs, layout_for_device, prepare,
                          source_amplitude, source_profiles)
 
 
+# Autotuned tile per grid signature (see Solver.autotune).
+_TILE_CACHE: Dict[tuple, tuple] = {}
+
+
 class DivergenceError(RuntimeError):
     """Raised by ``run`` when the reference's failure guards trip (driver.f90:175-183)."""
 
@@ -234,3 +238,28 @@ class Solver:
         nb = C.c_long()
         check(lib().sparc_hip_tile(self.h, t, C.byref(nb)), "sparc_hip_tile")
         return tuple(t), nb.value
+
+    def set_tile(self, tile: Sequence[int]):
+        t = (C.c_int * 3)(*[int(v) for v in tile])
+        check(lib().sparc_hip_set_tile(self.h, t), "sparc_hip_set_tile")
+
+    def tile_key(self) -> tuple:
+        """What the best tile depends on: grid, physics variant, precision,
+        boundary kind and CPML widths."""
+        c = self.cfg
+        return (c.nx, c.ny, c.nz, c.magnetic, c.precision, bool(c.periodic), bool(c.use_hpml),
+                tuple(c.npml), bool(self.st.bg3d))
+
+    def autotune(self, reps: int = 3, cache: bool = True):
+        """Pick the fastest tile shape for this grid by timing a middle RK
+        stage per candidate (the run's state is unchanged); runs with the same
+        tile_key reuse the answer.  Returns (tile, ms per stage or None)."""
+        key = self.tile_key()
+        if cache and key in _TILE_CACHE:
+            self.set_tile(_TILE_CACHE[key])
+            return _TILE_CACHE[key], None
+        t = (C.c_int * 3)()
+        ms = C.c_double()
+        check(lib().sparc_hip_autotune(self.h, int(reps), t, C.byref(ms)), "sparc_hip_autotune")
+        _TILE_CACHE[key] = tuple(t)
+        return tuple(t), ms.value

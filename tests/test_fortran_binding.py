@@ -40,3 +40,7 @@ def test_fortran_module_binds_every_entry_point():
     bound = set(re.findall(r'bind\(C,\s*name="(sparc_hip_\w+)"\)', f90))
     assert bound <= declared, bound - declared
     assert declared - bound <= {"sparc_hip_strerror"}, declared - bound
+    # ... and every interface is public (the module is private by default)
+    pub = re.search(r"^\s*public ::(.*?)\n\s*\n", f90, re.S | re.M).group(1)
+    public = set(re.findall(r"\b(sparc_\w+)", pub))
+    assert (bound - {"sparc_hip_last_error"}) <= public, (bound - {"sparc_hip_last_error"}) - public
