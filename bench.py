@@ -216,18 +216,30 @@ def mode_batch(args):
 
 def mode_serve(args):
     """The served form: a seeded Poisson stream of requests through the
-    continuous-batching server; runs/h, throughput and latency percentiles."""
+    continuous-batching server, FP32+Kahan and then the same stream in plain
+    FP64; runs/h, throughput and latency percentiles of each.  --step-scale
+    shrinks the drawn step counts (a bounded sample of the stream)."""
     from sparc2_amd.serve import Server, poisson_stream
-    reqs = poisson_stream(args.runs or 8)
-    r = Server().run(reqs, verbose=True)
+    n = args.runs or 8
+    lo, hi = (max(1, int(round(v * args.step_scale))) for v in (500, 5000))
+    out = {}
+    for prec in ("fp32", "fp64"):
+        reqs = poisson_stream(n, precision=prec, steps=(lo, hi))
+        r = Server().run(reqs, verbose=True)
+        out[prec] = {"runs_per_h": round(r["runs_per_h"], 2), "gpts": round(r["gpts"], 5),
+                     "gpts_busy": round(r["gpts_busy"], 5), "wall_s": round(r["wall_s"], 3),
+                     "latency_p50_s": round(r["latency_p50"], 3),
+                     "latency_p95_s": round(r["latency_p95"], 3),
+                     "latency_p99_s": round(r["latency_p99"], 3)}
+        log(f"[serve] {prec}: {out[prec]}")
     print(json.dumps({
-        "metric": METRIC, "mode": "serve", "value": round(r["runs_per_h"], 2), "unit": "runs/h",
+        "metric": METRIC, "mode": "serve", "value": out["fp32"]["runs_per_h"], "unit": "runs/h",
         "n_gpus": 1, "higher_is_better": True, "dtype": "f32+kahan",
         "data": "synthetic (seeded Poisson stream, rate 4/s)",
-        "config": {"workload": "poisson_stream", "requests": len(reqs)},
-        "gpts": round(r["gpts"], 5), "gpts_busy": round(r["gpts_busy"], 5),
-        "wall_s": round(r["wall_s"], 3), "latency_p50_s": round(r["latency_p50"], 3),
-        "latency_p95_s": round(r["latency_p95"], 3), "latency_p99_s": round(r["latency_p99"], 3)}),
+        "config": {"workload": "poisson_stream", "requests": n, "steps_drawn_from": [lo, hi],
+                   "grids": "64^3 .. 384x384x256"},
+        "fp32": out["fp32"], "fp64": out["fp64"],
+        "gpts_ratio_fp32_fp64": round(out["fp32"]["gpts_busy"] / out["fp64"]["gpts_busy"], 3)}),
           flush=True)
 
 
@@ -274,6 +286,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="batch: keep the default tiles")
     ap.add_argument("--fp64-steps", type=int, default=0,
                     help="batch: also time this many steps of every run alone in plain FP64")
+    ap.add_argument("--step-scale", type=float, default=1.0,
+                    help="serve: scale of the drawn step counts (500..5000 at 1)")
     ap.add_argument("--runs", type=int, default=None,
                     help="batch: number of runs (64); serve: number of requests (8)")
     args = ap.parse_args()

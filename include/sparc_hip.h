@@ -178,6 +178,10 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
 int sparc_hip_batch_advance(sparc_batch* b, int nsteps, const int* time0, const double* amps);
 int sparc_hip_batch_info(sparc_batch* b, int* ngroups, long* nlaunch_blocks);
 int sparc_hip_batch_destroy(sparc_batch* b);
+/* Time per middle RK stage of the whole batch (reps launches of every
+ * group), for tuning the tile shape of a group of runs; leaves every run's
+ * state unchanged.  No counterpart in the reference. */
+int sparc_hip_batch_time_stage(sparc_batch* b, int reps, double* ms_out);
 
 const char* sparc_hip_strerror(int code);
 const char* sparc_hip_last_error(void);

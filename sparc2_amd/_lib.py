@@ -57,7 +57,7 @@ EXPORTS = [
     "sparc_hip_kernel_time", "sparc_hip_tile", "sparc_hip_strerror", "sparc_hip_last_error",
     "sparc_hip_version", "sparc_hip_batch_create", "sparc_hip_batch_advance",
     "sparc_hip_batch_info", "sparc_hip_batch_destroy", "sparc_hip_alfven_search",
-    "sparc_hip_get_slab", "sparc_hip_autotune", "sparc_hip_set_tile",
+    "sparc_hip_get_slab", "sparc_hip_autotune", "sparc_hip_set_tile", "sparc_hip_batch_time_stage",
 ]
 
 _lib = None

@@ -11,12 +11,63 @@ states stay device-resident in their own contexts; ``Solver.get_state`` and
 from __future__ import annotations
 
 import ctypes as C
-from typing import List, Sequence
+from typing import Dict, List, Sequence
 
 import numpy as np
 
-from ._lib import check, dptr, lib
+from ._lib import SparcError, check, dptr, lib
 from .solver import Solver
+
+
+# Tile shapes the stage kernels are built for.
+TILES = ((16, 16), (32, 8), (32, 16), (64, 4), (64, 8))
+
+
+def _create(solvers: Sequence[Solver], use_graph: bool):
+    n = len(solvers)
+    hs = (C.c_void_p * n)(*[s.h.value for s in solvers])
+    h = C.c_void_p()
+    check(lib().sparc_hip_batch_create(hs, n, int(use_graph), C.byref(h)), "sparc_hip_batch_create")
+    return h
+
+
+def _time_stage(h, reps: int) -> float:
+    ms = C.c_double()
+    check(lib().sparc_hip_batch_time_stage(h, int(reps), C.byref(ms)), "sparc_hip_batch_time_stage")
+    return ms.value
+
+
+def tune_tiles(solvers: Sequence[Solver], reps: int = 3) -> Dict[tuple, tuple]:
+    """Tile autotuning for a batch.  Runs that share a kernel variant share
+    one launch, so the shape is chosen per variant group, by timing a middle
+    RK stage of that group's batched launch for every shape that fits all of
+    its grids (timing leaves the runs' states unchanged).  Returns
+    {variant: (tx, ty)} and leaves each run on its group's best shape."""
+    groups: Dict[tuple, List[Solver]] = {}
+    for s in solvers:
+        groups.setdefault((s.cfg.magnetic == 2, bool(s.st.bg3d)), []).append(s)
+    chosen = {}
+    for key, members in groups.items():
+        best, best_ms = None, float("inf")
+        for t in TILES:
+            try:
+                for s in members:
+                    s.set_tile((t[0], t[1], 0))
+            except SparcError:
+                continue
+            h = _create(members, use_graph=False)
+            try:
+                ms = _time_stage(h, reps)
+            finally:
+                lib().sparc_hip_batch_destroy(h)
+            if ms < best_ms:
+                best, best_ms = t, ms
+        if best is None:
+            raise SparcError(f"no tile shape fits every grid of variant {key}")
+        for s in members:
+            s.set_tile((best[0], best[1], 0))
+        chosen[key] = best
+    return chosen
 
 
 class Batch:
@@ -24,16 +75,13 @@ class Batch:
         self.solvers: List[Solver] = list(solvers)
         if not self.solvers:
             raise ValueError("empty batch")
-        if autotune:
-            # one timing sweep per distinct grid; runs alike reuse it
-            for s in self.solvers:
-                s.autotune()
-        n = len(self.solvers)
-        hs = (C.c_void_p * n)(*[s.h.value for s in self.solvers])
-        h = C.c_void_p()
-        check(lib().sparc_hip_batch_create(hs, n, int(use_graph), C.byref(h)), "sparc_hip_batch_create")
-        self.h = h
-        self.time = [0] * n      # next time index of each run
+        self.tiles = tune_tiles(self.solvers) if autotune else None
+        self.h = _create(self.solvers, use_graph)
+        self.time = [0] * len(self.solvers)      # next time index of each run
+
+    def time_stage(self, reps: int = 3) -> float:
+        """Milliseconds per middle RK stage of the whole batch."""
+        return _time_stage(self.h, reps)
 
     def info(self):
         ng = C.c_int()

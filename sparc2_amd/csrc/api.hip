@@ -1157,6 +1157,45 @@ int sparc_hip_batch_advance(sparc_batch* b, int nsteps, const int* time0, const 
   return b->f32 ? batch_advance<float>(b, nsteps, time0, amps) : batch_advance<double>(b, nsteps, time0, amps);
 }
 
+// Average time of one middle RK stage of the whole batch (every group's z
+// pass and stage launch with the stage-2 arguments).  Like the single-run
+// autotune timing it writes only buffers the next step overwrites.
+template <typename T>
+static int batch_time_stage_t(sparc_batch* b, int reps, double* ms) {
+  int rc = batch_upload_args<T>(b);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  hipError_t e = hipSuccess;
+  for (int q = 0; q <= reps && e == hipSuccess; ++q) {
+    if (q == 1) e = hipEventRecord(e0, b->st);
+    for (auto& g : b->groups) {
+      if (e != hipSuccess) break;
+      const StageArgs<T>* args = (const StageArgs<T>*)g.args[1];
+      const int nm = (int)g.members.size();
+      if (g.zp) e = launch_zpass_batch<T>(args, g.zitem_start, nm, g.znblk, b->st);
+      if (e == hipSuccess)
+        e = launch_stage_batch<T>(g.mhd, g.bg3d, g.zp, g.tx, g.ty, args, g.item_start, nm, g.nblk, b->st);
+    }
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, b->st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch timing: ") + hipGetErrorString(e));
+  *ms = t / reps;
+  return SPARC_OK;
+}
+
+int sparc_hip_batch_time_stage(sparc_batch* b, int reps, double* ms_out) {
+  if (!b || !ms_out || reps < 1) return fail(SPARC_E_INVALID, "bad argument");
+  HIPCHK(hipSetDevice(b->dev));
+  return b->f32 ? batch_time_stage_t<float>(b, reps, ms_out) : batch_time_stage_t<double>(b, reps, ms_out);
+}
+
 int sparc_hip_batch_info(sparc_batch* b, int* ngroups, long* nlaunch_blocks) {
   if (!b || !ngroups || !nlaunch_blocks) return fail(SPARC_E_INVALID, "null argument");
   *ngroups = (int)b->groups.size();

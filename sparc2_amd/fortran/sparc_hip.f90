@@ -55,7 +55,7 @@ module sparc_hip
        sparc_hip_timing, sparc_hip_kernel_time, sparc_hip_tile, sparc_hip_version, &
        sparc_hip_alfven_search, sparc_hip_get_slab, sparc_hip_batch_create, &
        sparc_hip_batch_advance, sparc_hip_batch_info, sparc_hip_batch_destroy, &
-       sparc_hip_autotune, sparc_hip_set_tile, sparc_error_message
+       sparc_hip_autotune, sparc_hip_set_tile, sparc_hip_batch_time_stage, sparc_error_message
 
   interface
     integer(c_int) function sparc_hip_create(cfg, ctx) bind(C, name="sparc_hip_create")
@@ -231,6 +231,14 @@ module sparc_hip
       type(c_ptr), value :: batch
       integer(c_int), intent(out) :: ngroups
       integer(c_long), intent(out) :: nlaunch_blocks
+    end function
+
+    integer(c_int) function sparc_hip_batch_time_stage(batch, reps, ms_out) &
+         bind(C, name="sparc_hip_batch_time_stage")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int), value :: reps
+      real(c_double), intent(out) :: ms_out
     end function
 
     integer(c_int) function sparc_hip_batch_destroy(batch) bind(C, name="sparc_hip_batch_destroy")

@@ -36,8 +36,9 @@ def _solvers(cases):
     return ss
 
 
-@pytest.mark.parametrize("precision,use_graph", [("fp64", False), ("fp64", True), ("fp32", True)])
-def test_batch_equals_alone(precision, use_graph):
+@pytest.mark.parametrize("precision,use_graph,autotune", [("fp64", False, False), ("fp64", True, False),
+                                                         ("fp32", True, False), ("fp32", True, True)])
+def test_batch_equals_alone(precision, use_graph, autotune):
     from sparc2_amd.batch import Batch
     cases = _cases(precision)
     nsteps = 13
@@ -47,8 +48,8 @@ def test_batch_equals_alone(precision, use_graph):
         alone.append(s.get_state(with_psi=True))
         s.close()
     ss = _solvers(cases)
-    b = Batch(ss, use_graph=use_graph)
-    print("groups, blocks:", b.info(), flush=True)
+    b = Batch(ss, use_graph=use_graph, autotune=autotune)
+    print("groups, blocks:", b.info(), "tiles:", b.tiles, flush=True)
     b.advance(5)          # two calls: the second continues the time index
     b.advance(nsteps - 5)
     for q, s in enumerate(ss):
