@@ -23,11 +23,15 @@ namespace sparc {
 
 // Occupancy target (waves per SIMD): the fused MHD kernel's register queue (6
 // fields x 11 planes) plus nine LDS-staged fields need about 250 registers,
-// so it runs 2 waves per SIMD; the quiet variants and the MHD xy kernel (z
-// derivatives from the z pass) are held to 3 so memory latency overlaps.
-template <typename T, bool MHD, bool ZP>
+// so it runs 2 waves per SIMD; the MHD xy kernel (z derivatives from the z
+// pass) is held to 3.  The quiet kernel keeps the next plane's values in
+// registers, which fits 3 waves only in FP32 without 3-D backgrounds.
+template <typename T, bool MHD, bool BG3D, bool ZP>
 constexpr int stage_waves() {
-  return (MHD && !ZP) ? 2 : 3;
+  if (MHD) return ZP ? 3 : 2;
+  // quiet with the one-plane-ahead point values: FP64, or FP32 with 3-D
+  // backgrounds, need more registers than 3 waves per SIMD allow
+  return (sizeof(T) == 8 || BG3D) ? 2 : 3;
 }
 
 // One work item: tile `tile` of z chunk `chunk` (logical = chunk*nblk_xy + tile)
@@ -46,10 +50,15 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   // LDS planes: 0 vx (x), 1 vy (y), 2 p (xy); MHD: 3 F3 (xy), 4 F2 (x),
   // 5 F1 (y), 6 by (x), 7 bz (xy), 8 bx (y)
   constexpr int NL = MHD ? 9 : 3;
-  // Quiet: two LDS plane buffers, the next plane's staging loads issued
-  // before the current plane's arithmetic (one barrier per plane).  MHD: one
-  // buffer (two would halve the blocks per CU), loads then barrier per plane.
-  constexpr int NB = MHD ? 1 : 2;
+  // SPF (quiet): two LDS plane buffers, the next plane's staging loads held
+  // in registers while the current plane computes (one barrier per plane).
+  // MHD: one buffer, loads then a barrier per plane -- with nine staged
+  // planes a second buffer halves the blocks per CU, which costs more than
+  // the prefetch hides (measured in FP32: 1.32 -> 1.37 ms at 32x16,
+  // 1.59 -> 1.88 ms at 16x16 on 256^3).
+  constexpr bool SPF = !MHD;
+  constexpr int NB = SPF ? 2 : 1;
+  constexpr int NPRE = MHD ? 10 : 3;  // raw values staged per cell
   // one spare cell per plane takes the writes of slots past the region
   constexpr int LP = LY * LX + 1;
   __shared__ T LB[NB][NL][LP];
@@ -121,8 +130,9 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   auto flux2 = [&](uint32_t g) -> T { return -UF(VX, g) * B3(Ap->boz, g) + UF(VZ, g) * B3(Ap->box, g); };
 
   T(*L)[LP] = LB[0];  // the plane buffer in use
-  // quiet: staging values of the next plane, held in registers
-  T pre[NS][3];
+  // SPF: staging values of the next plane, held in registers (MHD: v, p,
+  // b and B0; the fluxes are formed when the plane is written to LDS)
+  T pre[NS][NPRE];
   auto pre_load = [&](int k) {
     const uint32_t pk = (uint32_t)k * nxy;
 #pragma unroll
@@ -131,15 +141,34 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       pre[s][0] = UFb(VX, bo);
       pre[s][1] = UFb(VY, bo);
       pre[s][2] = UFb(P, bo);
+      if constexpr (MHD) {
+        pre[s][3] = UFb(VZ, bo);
+        pre[s][4] = UFb(BX, bo);
+        pre[s][5] = UFb(BY, bo);
+        pre[s][6] = UFb(BZ, bo);
+        pre[s][7] = B3b(Ap->box, bo);
+        pre[s][8] = B3b(Ap->boy, bo);
+        pre[s][9] = B3b(Ap->boz, bo);
+      }
     }
   };
   auto pre_store = [&]() {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const int c = s_lds[s];
-      L[0][c] = pre[s][0];
-      L[1][c] = pre[s][1];
+      const T vx = pre[s][0], vy = pre[s][1];
+      L[0][c] = vx;
+      L[1][c] = vy;
       L[2][c] = pre[s][2];
+      if constexpr (MHD) {
+        const T vz = pre[s][3], b0x = pre[s][7], b0y = pre[s][8], b0z = pre[s][9];
+        L[3][c] = -vy * b0x + vx * b0y;  // F3 = -vy box + vx boy
+        L[4][c] = -vx * b0z + vz * b0x;  // F2
+        L[5][c] = -vz * b0y + vy * b0z;  // F1
+        L[6][c] = pre[s][5];
+        L[7][c] = pre[s][6];
+        L[8][c] = pre[s][4];
+      }
     }
   };
   auto stage_plane = [&](int k) {
@@ -241,16 +270,19 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     else return ((CPtr<T>)p)[k];
   };
 
-  // Quiet FP32: the point values of a plane -- rho, the RK base a, and the 3-D
-  // backgrounds where present -- are loaded one plane ahead too, so every load
-  // a plane needs was issued a whole iteration earlier and the iteration
-  // waits once, at its top.  (In-order completion counting would otherwise
-  // make the point loads wait for the next plane's staging loads as well.)
-  // In FP64 the extra registers would cost a wave per SIMD, so FP64 loads the
-  // point values in the plane itself.
-  constexpr bool PPF = !MHD && sizeof(T) == 4;
+  // Quiet: the point values of a plane -- rho, the RK base a, the 3-D
+  // backgrounds where present, and on the last stage the Kahan compensation
+  // (FP32) and p0 -- are loaded one plane ahead too, so every load a plane
+  // needs was issued a whole iteration earlier and the iteration waits once,
+  // at its top.  (In-order completion counting would otherwise make the point
+  // loads wait for the next plane's staging loads as well.)  FP64 runs this
+  // at 2 waves per SIMD (its registers would not fit 3).
+  constexpr bool PPF = !MHD;
   constexpr int NPB = (PPF && BG3D) ? 6 : 0;  // rho0, rhoinv, sponge, gr0z, c2rho0, gp0z
-  constexpr int NP = PPF ? 1 + NV + NPB : 1;
+  constexpr int NPC = (PPF && sizeof(T) == 4) ? NV : 0;  // comp[f], last stage
+  constexpr int NPP = (PPF && BG3D) ? 1 : 0;             // p0, last stage
+  constexpr int IC = 1 + NV + NPB, IP = IC + NPC;         // slots of comp and p0
+  constexpr int NP = PPF ? IP + NPP : 1;
   T pc[NP], pn[NP];
   auto point_load = [&](int kk, T* dst) {
     if constexpr (PPF) {
@@ -266,14 +298,21 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         dst[1 + NV + 4] = B3b(Ap->c2rho0, bo);
         dst[1 + NV + 5] = B3b(Ap->gp0z, bo);
       }
+      if (last) {
+        if constexpr (NPC > 0) {
+#pragma unroll
+          for (int f = 0; f < NV; ++f) dst[IC + f] = Buf<T>(Ap->comp[f], fs).ldb(bo);
+        }
+        if constexpr (NPP > 0) dst[IP] = B3b(Ap->p0, bo);
+      }
     }
   };
-  if constexpr (!MHD) pre_load(k0);
+  if constexpr (SPF) pre_load(k0);
   if constexpr (PPF) point_load(k0, pn);
   for (int k = k0; k < k1; ++k) {
     SPARC_REFRESH(Ap);
     L = LB[NB == 2 ? (k & 1) : 0];
-    if constexpr (!MHD) {
+    if constexpr (SPF) {
       pre_store();
       if constexpr (PPF) {
 #pragma unroll
@@ -494,7 +533,9 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           if constexpr (sizeof(T) == 4) {
             // compensated update: a - c carries the bits a + dt K loses
             const Buf<T> cf(Ap->comp[f], fs);
-            const T c0 = cf[g];
+            T c0;
+            if constexpr (NPC > 0) c0 = pc[IC + f];
+            else c0 = cf[g];
             const T y = dt * K[f] - c0;
             const T t = a0 + y;
             cf.st(g, (t - a0) - y);
@@ -510,7 +551,10 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
                          2.0;
         ke += m * m;
         mr = fmin(mr, (double)rho + (double)rho0);
-        mp = fmin(mp, (double)p + (double)BG(Ap->p0, g, k));
+        T p0v;
+        if constexpr (NPP > 0) p0v = pc[IP];
+        else p0v = BG(Ap->p0, g, k);
+        mp = fmin(mp, (double)p + (double)p0v);
       }
     }
     // single buffer: the next plane's staging must wait for these reads
@@ -696,7 +740,7 @@ __global__ __launch_bounds__(ZP_NT, 2) void k_zpass_batch(const StageArgs<T>* __
 // ids to the 8 XCDs in turn; remap so each XCD gets a contiguous run of tiles
 // and the halos the tiles share stay in that XCD's L2.
 template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
-__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, ZP>())) void k_stage(const StageArgs<T> A) {
+__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP>())) void k_stage(const StageArgs<T> A) {
   const int nblk = gridDim.x, b = blockIdx.x;
   const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
   const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
@@ -709,7 +753,7 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, ZP>())) void k_stage(c
 // XCD takes a contiguous eighth of the list and its blocks stride through it.
 // The grid size must be a multiple of 8.
 template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
-__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, ZP>())) void k_stage_batch(
+__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP>())) void k_stage_batch(
     const StageArgs<T>* __restrict__ runs, const int* __restrict__ item_start, int nrun) {
   const int total = item_start[nrun];
   const int xcd = blockIdx.x & 7, m = blockIdx.x >> 3, nb8 = gridDim.x >> 3;
