@@ -143,7 +143,7 @@ int sparc_hip_tile(sparc_ctx* ctx, int tile[3], long* nblocks);
 /* Tile autotuner (no counterpart in the reference; it serves the batched and
  * served forms, whose runs differ in grid size).  Times `reps` launches of a
  * middle RK stage for every supported tile shape that fits the grid, at the
- * automatic z chunk and twice it, and keeps the fastest: tile_out = (tx, ty,
+ * automatic z chunk, half and twice it, and keeps the fastest: tile_out = (tx, ty,
  * kc), ms_out = its time per stage.  A middle stage writes only buffers the
  * next step overwrites, so the run's state is unchanged.  Call between steps
  * after set_background and put_state.  sparc_hip_set_tile applies a known

@@ -197,13 +197,12 @@ static void free_all(sparc_ctx* c) {
 
 // Tile choice.  Shapes the stage kernel is built for, in order of preference
 // per variant (measured on MI355X, 512x512x300 quiet and 256^3 MHD: quiet
-// FP64 and FP32 run fastest at 32x8 -- three 256-thread blocks per CU -- the
-// MHD xy kernel at 16x16 in FP64 (a 512-thread tile fits only one block per
-// CU) and at 32x16 in FP32 (two fit)).
+// FP64 and FP32 run fastest at 32x8, the MHD xy kernel at 32x16 in FP64 and
+// at 32x8 in FP32).
 // The first shape that fits the grid (one-sided closures need >= 6 rows in
 // the last tile of a non-periodic axis) is taken.  The z chunk is sized so
-// the launch has >= ~2048 blocks (8 per CU on 256 CUs) without dropping below
-// 4 planes per block.
+// the launch has >= ~2048 blocks (quiet) or ~8192 (MHD xy kernel) without
+// dropping below 4 planes per block.
 static bool tile_fits(const sparc_ctx* c, int tx, int ty) {
   if (!stage_tile_supported(tx, ty)) return false;
   if (c->cfg.periodic) return true;
@@ -219,8 +218,8 @@ static void choose_tile(sparc_ctx* c) {
   if (!(tx > 0 && ty > 0 && stage_tile_supported(tx, ty))) {
     static const int quiet64[4][2] = {{32, 8}, {16, 16}, {32, 16}, {64, 4}};
     static const int quiet32[4][2] = {{32, 8}, {64, 4}, {16, 16}, {32, 16}};
-    static const int mhd64[4][2] = {{16, 16}, {32, 8}, {32, 16}, {64, 4}};
-    static const int mhd32[4][2] = {{32, 16}, {32, 8}, {16, 16}, {64, 4}};
+    static const int mhd64[4][2] = {{32, 16}, {32, 8}, {16, 16}, {64, 4}};
+    static const int mhd32[4][2] = {{32, 8}, {32, 16}, {16, 16}, {64, 4}};
     const int(*pref)[2] = c->mhd ? (c->f32 ? mhd32 : mhd64) : (c->f32 ? quiet32 : quiet64);
     tx = 32;
     ty = 8;
@@ -245,7 +244,10 @@ static void set_geometry(sparc_ctx* c, int tx, int ty, int kc) {
   c->nty = (c->ny + ty - 1) / ty;
   long tiles = (long)c->ntx * c->nty;
   if (kc <= 0) {
-    long want = 2048;
+    // the quiet kernel re-reads 10 planes of its z queue per chunk, so it
+    // keeps chunks deep (~2048 blocks); the MHD xy kernel has no queue and
+    // runs fastest with many shallow chunks (~8192 blocks, measured)
+    long want = c->zp ? 8192 : 2048;
     kc = (int)std::max<long>(4, (c->nz * tiles + want - 1) / want);
     kc = std::min(kc, c->nz);
   }
@@ -845,11 +847,14 @@ int sparc_hip_autotune(sparc_ctx* c, int reps, int tile_out[3], double* ms_out) 
   double best_ms = 1e30;
   for (const auto& sh : shapes) {
     if (!tile_fits(c, sh[0], sh[1])) continue;
-    for (int mult = 1; mult <= 2; ++mult) {
-      int rc = retile(c, sh[0], sh[1], 0);
-      if (rc) return rc;
-      const int kc = std::min(c->nz, c->kc * mult);
-      if (mult == 2 && kc == c->kc) continue;
+    int rc = retile(c, sh[0], sh[1], 0);
+    if (rc) return rc;
+    const int kauto = c->kc;
+    // the automatic z chunk, half of it (more blocks) and twice it
+    const int kcs[3] = {kauto, std::max(4, kauto / 2), std::min(c->nz, kauto * 2)};
+    for (int q = 0; q < 3; ++q) {
+      const int kc = kcs[q];
+      if (q > 0 && kc == kauto) continue;
       if ((rc = retile(c, sh[0], sh[1], kc))) return rc;
       double ms = 0.0;
       rc = c->f32 ? time_middle_stage<float>(c, reps, &ms) : time_middle_stage<double>(c, reps, &ms);

@@ -28,7 +28,7 @@ namespace sparc {
 // registers, which fits 3 waves only in FP32 without 3-D backgrounds.
 template <typename T, bool MHD, bool BG3D, bool ZP>
 constexpr int stage_waves() {
-  if (MHD) return ZP ? 3 : 2;
+  if (MHD) return (ZP && sizeof(T) == 4) ? 3 : 2;
   // quiet with the one-plane-ahead point values: FP64, or FP32 with 3-D
   // backgrounds, need more registers than 3 waves per SIMD allow
   return (sizeof(T) == 8 || BG3D) ? 2 : 3;
@@ -307,6 +307,40 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       }
     }
   };
+  // MHD xy kernel: a plane's staging loads are issued, then its point values
+  // (rho, vz, the six z derivatives, a, the 3-D backgrounds; p0 and the FP32
+  // compensation on the last stage), then the wait covers the staging loads
+  // only; the LDS write and barrier overlap the point loads' latency.
+  constexpr bool MPX = MHD && ZP;
+  constexpr int MP_A = 8, MP_BG = 16, MP_P0 = 33, MP_C = 34;
+  constexpr int NPM = MPX ? MP_C + (sizeof(T) == 4 ? 8 : 0) : 1;
+  T pm[NPM];
+  auto mpoint_load = [&](int kk) {
+    if constexpr (MPX) {
+      const uint32_t bo = active ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
+      pm[0] = UFb(RHO, bo);
+      pm[1] = UFb(VZ, bo);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) pm[2 + f] = Buf<T>(Ap->dz[f], fs).ldb(bo);
+#pragma unroll
+      for (int f = 0; f < 8; ++f) pm[MP_A + f] = Buf<T>(Ap->a[f], fs).ldb(bo);
+      const T* const bgp[17] = {Ap->rho0, Ap->rhoinv, Ap->sponge, Ap->red, Ap->box, Ap->boy,
+                                Ap->boz, Ap->gr0x, Ap->gr0y, Ap->gr0z, Ap->c2rho0, Ap->gp0x,
+                                Ap->gp0y, Ap->gp0z, Ap->cbox, Ap->cboy, Ap->cboz};
+#pragma unroll
+      for (int q2 = 0; q2 < 17; ++q2) pm[MP_BG + q2] = B3b(bgp[q2], bo);
+      if (last) {
+        pm[MP_P0] = B3b(Ap->p0, bo);
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+          for (int f = 0; f < 8; ++f) pm[MP_C + f] = Buf<T>(Ap->comp[f], fs).ldb(bo);
+        }
+      }
+    }
+  };
+  // background slot at the point in pm (MHD xy kernel)
+  auto PMB = [&](int q2) -> T { return pm[MP_BG + q2]; };
+
   if constexpr (SPF) pre_load(k0);
   if constexpr (PPF) point_load(k0, pn);
   for (int k = k0; k < k1; ++k) {
@@ -322,6 +356,10 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         pre_load(k + 1);
         if constexpr (PPF) point_load(k + 1, pn);
       }
+    } else if constexpr (MPX) {
+      pre_load(k);
+      mpoint_load(k);
+      pre_store();
     } else {
       stage_plane(k);
     }
@@ -373,10 +411,10 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       T dvxdx = dx_lds(0), dvydy = dy_lds(1), gpx = dx_lds(2), gpy = dy_lds(2);
       const int cc = (ty + 5) * LX + tx + 5;
       const T vx = L[0][cc], vy = L[1][cc];
-      const T rho = PPF ? pc[0] : UF(RHO, g);
-      const T vz = ZP ? UF(VZ, g) : q[0][5], p = ZP ? L[2][cc] : q[1][5];
+      const T rho = PPF ? pc[0] : (MPX ? pm[0] : UF(RHO, g));
+      const T vz = ZP ? (MPX ? pm[1] : UF(VZ, g)) : q[0][5], p = ZP ? L[2][cc] : q[1][5];
       // z pass output, in CPML family order: vz, p, bx, by, F2, F1
-      auto DZ = [&](int f) -> T { return Buf<T>(Ap->dz[f], fs)[g]; };
+      auto DZ = [&](int f) -> T { return MPX ? pm[2 + f] : Buf<T>(Ap->dz[f], fs)[g]; };
       T dvzdz = ZP ? DZ(0) : dz[0], gpz = ZP ? DZ(1) : dz[1];
       T dxf3 = 0, dyf3 = 0, dxf2 = 0, dyf1 = 0, dxby = 0, dxbz = 0, dybz = 0, dybx = 0, bz = 0;
       T dzbx = 0, dzby = 0, dzf1 = 0, dzf2 = 0;
@@ -464,6 +502,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       // background at the point (quiet 3-D backgrounds come from the prefetch)
       auto BGP = [&](const T* p, int slot) -> T {
         if constexpr (NPB > 0) return pc[1 + NV + slot];
+        else if constexpr (MPX) return PMB(slot);  // rho0, rhoinv, sponge
         else return BG(p, g, k);
       };
       const T rho0 = BGP(Ap->rho0, 0), rhoinv = BGP(Ap->rhoinv, 1), sp = BGP(Ap->sponge, 2);
@@ -478,21 +517,25 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         K[VZ] = -rhoinv * (gpz + rho * gk) - vz * sp;
         K[P] = -BGP(Ap->c2rho0, 4) * div - vz * BGP(Ap->gp0z, 5) - sp * p;
       } else {
-        const T red = B3(Ap->red, g);
-        const T b0x = B3(Ap->box, g), b0y = B3(Ap->boy, g), b0z = B3(Ap->boz, g);
+        const T red = MPX ? PMB(3) : B3(Ap->red, g);
+        const T b0x = MPX ? PMB(4) : B3(Ap->box, g), b0y = MPX ? PMB(5) : B3(Ap->boy, g),
+                b0z = MPX ? PMB(6) : B3(Ap->boz, g);
         const T bx = ZP ? L[8][cc] : q[2][5], by = ZP ? L[6][cc] : q[3][5];
         // RHS.f90:548-580
         K[BX] = red * (dyf3 - dzf2);
         K[BY] = red * (dzf1 - dxf3);
         K[BZ] = red * (dxf2 - dyf1);
         const T cbx = dybz - dzby, cby = -dxbz + dzbx, cbz = dxby - dybx;
-        K[RHO] = -B3(Ap->gr0x, g) * vx - B3(Ap->gr0y, g) * vy - B3(Ap->gr0z, g) * vz - rho0 * div -
-                 sp * rho;
-        K[P] = -B3(Ap->c2rho0, g) * div - vx * B3(Ap->gp0x, g) - vy * B3(Ap->gp0y, g) -
-               vz * B3(Ap->gp0z, g) - sp * p;
+        const T gr0x = MPX ? PMB(7) : B3(Ap->gr0x, g), gr0y = MPX ? PMB(8) : B3(Ap->gr0y, g),
+                gr0z = MPX ? PMB(9) : B3(Ap->gr0z, g), c2r = MPX ? PMB(10) : B3(Ap->c2rho0, g),
+                gp0x = MPX ? PMB(11) : B3(Ap->gp0x, g), gp0y = MPX ? PMB(12) : B3(Ap->gp0y, g),
+                gp0z = MPX ? PMB(13) : B3(Ap->gp0z, g);
+        K[RHO] = -gr0x * vx - gr0y * vy - gr0z * vz - rho0 * div - sp * rho;
+        K[P] = -c2r * div - vx * gp0x - vy * gp0y - vz * gp0z - sp * p;
         // CROSS(curl b, B0) and CROSS(curl B0, b), derivatives.f90:646-655
         const T c1 = cby * b0z - cbz * b0y, c2 = b0x * cbz - cbx * b0z, c3 = cbx * b0y - cby * b0x;
-        const T cox = B3(Ap->cbox, g), coy = B3(Ap->cboy, g), coz = B3(Ap->cboz, g);
+        const T cox = MPX ? PMB(14) : B3(Ap->cbox, g), coy = MPX ? PMB(15) : B3(Ap->cboy, g),
+                coz = MPX ? PMB(16) : B3(Ap->cboz, g);
         const T e1 = coy * bz - coz * by;
         const T e2 = bx * coz - cox * bz;
         const T e3 = cox * by - coy * bx;
@@ -521,7 +564,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
           const Buf<T> uf(Ap->un[f], fs);
-          const T a0 = PPF ? pc[1 + f] : Buf<T>(Ap->a[f], fs)[g];
+          const T a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : Buf<T>(Ap->a[f], fs)[g]);
           uf.st(g, a0 + bnext * K[f]);
         }
       } else {
@@ -529,12 +572,13 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
           const Buf<T> af(Ap->a[f], fs);
-          const T a0 = PPF ? pc[1 + f] : af[g];
+          const T a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : af[g]);
           if constexpr (sizeof(T) == 4) {
             // compensated update: a - c carries the bits a + dt K loses
             const Buf<T> cf(Ap->comp[f], fs);
             T c0;
             if constexpr (NPC > 0) c0 = pc[IC + f];
+            else if constexpr (MPX) c0 = pm[MP_C + f];
             else c0 = cf[g];
             const T y = dt * K[f] - c0;
             const T t = a0 + y;
@@ -553,6 +597,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         mr = fmin(mr, (double)rho + (double)rho0);
         T p0v;
         if constexpr (NPP > 0) p0v = pc[IP];
+        else if constexpr (MPX) p0v = pm[MP_P0];
         else p0v = BG(Ap->p0, g, k);
         mp = fmin(mp, (double)p + (double)p0v);
       }
