@@ -19,7 +19,7 @@
 
 namespace sparc {
 template <typename T>
-hipError_t launch_stage(bool mhd, bool bg3d, bool zp, int tx, int ty, const StageArgs<T>& A,
+hipError_t launch_stage(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, const StageArgs<T>& A,
                         hipStream_t s);
 template <typename T>
 hipError_t launch_zpass(const StageArgs<T>& A, hipStream_t s);
@@ -37,9 +37,10 @@ hipError_t launch_deriv(int axis, const double* in, double* out, int nx, int ny,
 hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, hipStream_t st);
 hipError_t launch_zero(void* p, long n, bool f32, hipStream_t st);
 template <typename T>
-hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int tx, int ty, const StageArgs<T>* runs,
-                              const int* item_start, int nrun, int nblk, hipStream_t s);
-int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int tx, int ty, int ncu);
+hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty,
+                              const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
+                              hipStream_t s);
+int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, int ncu);
 hipError_t launch_diag_reduce_batch(const void* jobs, int njobs, hipStream_t st);
 hipError_t launch_tick(int* c, int reset, hipStream_t st);
 hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st);
@@ -102,6 +103,10 @@ struct sparc_ctx {
   // MHD: the z pass writes the six z derivatives here (6 slabs); zp is off
   // only for the fused single-kernel MHD stage (SPARC_MHD_FUSED=1)
   bool zp = false;
+  // x points per lane of the stage kernel: 2 for FP32 on even nx (quiet or
+  // MHD xy kernel; SPARC_LANE_POINTS=1 keeps one), else 1.  Tile widths
+  // count lanes, so a tile spans vw*tx points.
+  int vw = 1;
   void* dzbuf = nullptr;
   void* bg[B_COUNT] = {};
   void *g = nullptr, *stretch = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
@@ -195,10 +200,10 @@ static void free_all(sparc_ctx* c) {
   if (c->st) (void)hipStreamDestroy(c->st);
 }
 
-// Tile choice.  Shapes the stage kernel is built for, in order of preference
-// per variant (measured on MI355X, 512x512x300 quiet and 256^3 MHD: quiet
-// FP64 and FP32 run fastest at 32x8, the MHD xy kernel at 32x16 in FP64 and
-// at 32x8 in FP32).
+// Tile choice.  Shapes (in lanes) the stage kernel is built for, in order of
+// preference per variant (measured on MI355X, 512x512x300 quiet and 256^3
+// MHD: quiet FP64 and FP32 run fastest at 32x8, the MHD xy kernel at 32x16
+// in FP64, at 32x8 in FP32 with one point per lane and at 16x16 with two).
 // The first shape that fits the grid (one-sided closures need >= 6 rows in
 // the last tile of a non-periodic axis) is taken.  The z chunk is sized so
 // the launch has >= ~2048 blocks (quiet) or ~8192 (MHD xy kernel) without
@@ -206,7 +211,7 @@ static void free_all(sparc_ctx* c) {
 static bool tile_fits(const sparc_ctx* c, int tx, int ty) {
   if (!stage_tile_supported(tx, ty)) return false;
   if (c->cfg.periodic) return true;
-  const int rx = c->nx % tx, ry = c->ny % ty;
+  const int rx = c->nx % (tx * c->vw), ry = c->ny % ty;
   return ty >= 6 && !(rx > 0 && rx < 6) && !(ry > 0 && ry < 6);
 }
 
@@ -220,12 +225,15 @@ static void choose_tile(sparc_ctx* c) {
     static const int quiet32[4][2] = {{32, 8}, {64, 4}, {16, 16}, {32, 16}};
     static const int mhd64[4][2] = {{32, 16}, {32, 8}, {16, 16}, {64, 4}};
     static const int mhd32[4][2] = {{32, 8}, {32, 16}, {16, 16}, {64, 4}};
-    const int(*pref)[2] = c->mhd ? (c->f32 ? mhd32 : mhd64) : (c->f32 ? quiet32 : quiet64);
+    // two points per lane (FP32, even nx): the MHD xy kernel at 16x16 lanes
+    static const int mhd32x2[4][2] = {{16, 16}, {32, 8}, {32, 16}, {64, 4}};
+    const int(*pref)[2] = c->mhd ? (c->f32 ? (c->vw == 2 ? mhd32x2 : mhd32) : mhd64)
+                                 : (c->f32 ? quiet32 : quiet64);
     tx = 32;
     ty = 8;
     for (int n = 0; n < 4; ++n) {
       // narrow grids: a 16-wide tile wastes less of each block
-      if (c->nx <= 16 && pref[n][0] > 16) continue;
+      if (c->nx <= 16 && pref[n][0] * c->vw > 16) continue;
       if (tile_fits(c, pref[n][0], pref[n][1])) {
         tx = pref[n][0];
         ty = pref[n][1];
@@ -240,7 +248,7 @@ static void choose_tile(sparc_ctx* c) {
 static void set_geometry(sparc_ctx* c, int tx, int ty, int kc) {
   c->tx = tx;
   c->ty = ty;
-  c->ntx = (c->nx + tx - 1) / tx;
+  c->ntx = (c->nx + tx * c->vw - 1) / (tx * c->vw);
   c->nty = (c->ny + ty - 1) / ty;
   long tiles = (long)c->ntx * c->nty;
   if (kc <= 0) {
@@ -325,6 +333,8 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
   {
     const char* fused = getenv("SPARC_MHD_FUSED");
     c->zp = c->mhd && !(fused && fused[0] == '1');
+    const char* lp = getenv("SPARC_LANE_POINTS");
+    c->vw = (c->f32 && q.nx % 2 == 0 && (c->zp || !c->mhd) && !(lp && lp[0] == '1')) ? 2 : 1;
   }
   c->nv = c->mhd ? 8 : 5;
   c->npsi = c->mhd ? 6 : 2;
@@ -625,7 +635,7 @@ static int do_step(sparc_ctx* c, double amp) {
       if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("z pass launch: ") + hipGetErrorString(e));
     }
     timed_begin(c, A.first ? 0 : (A.last ? 2 : 1), &r);
-    hipError_t e = launch_stage<T>(c->mhd, c->bg3d, c->zp, c->tx, c->ty, A, c->st);
+    hipError_t e = launch_stage<T>(c->mhd, c->bg3d, c->zp, c->vw, c->tx, c->ty, A, c->st);
     timed_end(c, &r);
     if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("stage launch: ") + hipGetErrorString(e));
   }
@@ -824,7 +834,7 @@ static int time_middle_stage(sparc_ctx* c, int reps, double* ms) {
   for (int q = 0; q <= reps && e == hipSuccess; ++q) {
     if (q == 1) e = hipEventRecord(e0, c->st);
     if (e == hipSuccess && c->zp) e = launch_zpass<T>(A, c->st);
-    if (e == hipSuccess) e = launch_stage<T>(c->mhd, c->bg3d, c->zp, c->tx, c->ty, A, c->st);
+    if (e == hipSuccess) e = launch_stage<T>(c->mhd, c->bg3d, c->zp, c->vw, c->tx, c->ty, A, c->st);
   }
   if (e == hipSuccess) e = hipEventRecord(e1, c->st);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -897,7 +907,7 @@ int sparc_hip_tile(sparc_ctx* c, int tile[3], long* nblocks) {
 namespace {
 struct BatchGroup {
   bool mhd = false, bg3d = false, zp = false;
-  int tx = 0, ty = 0, nblk = 0, znblk = 0;
+  int vw = 1, tx = 0, ty = 0, nblk = 0, znblk = 0;
   std::vector<int> members;
   void* args[5] = {};         // StageArgs<T>[members] per stage (device)
   int* item_start = nullptr;   // prefix of stage work items (device)
@@ -963,7 +973,8 @@ static int batch_enqueue_step(sparc_batch* b) {
       hipError_t e = hipSuccess;
       if (g.zp) e = launch_zpass_batch<T>(args, g.zitem_start, nm, g.znblk, b->st);
       if (e == hipSuccess)
-        e = launch_stage_batch<T>(g.mhd, g.bg3d, g.zp, g.tx, g.ty, args, g.item_start, nm, g.nblk, b->st);
+        e = launch_stage_batch<T>(g.mhd, g.bg3d, g.zp, g.vw, g.tx, g.ty, args, g.item_start, nm, g.nblk,
+                                  b->st);
       if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch stage launch: ") + hipGetErrorString(e));
     }
   if (launch_diag_reduce_batch(b->jobs, (int)b->runs.size(), b->st) != hipSuccess ||
@@ -1040,13 +1051,16 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
     sparc_ctx* c = b->runs[r];
     BatchGroup* g = nullptr;
     for (auto& h : b->groups)
-      if (h.mhd == c->mhd && h.bg3d == c->bg3d && h.zp == c->zp && h.tx == c->tx && h.ty == c->ty) g = &h;
+      if (h.mhd == c->mhd && h.bg3d == c->bg3d && h.zp == c->zp && h.vw == c->vw && h.tx == c->tx &&
+          h.ty == c->ty)
+        g = &h;
     if (!g) {
       b->groups.emplace_back();
       g = &b->groups.back();
       g->mhd = c->mhd;
       g->bg3d = c->bg3d;
       g->zp = c->zp;
+      g->vw = c->vw;
       g->tx = c->tx;
       g->ty = c->ty;
     }
@@ -1065,7 +1079,7 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
     if ((rc = dalloc((void**)&g.item_start, sizeof(int) * start.size()))) return bail(rc);
     if (hipMemcpy(g.item_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(SPARC_E_HIP, "item table upload failed"));
-    g.nblk = std::min(stage_batch_blocks(b->f32, g.mhd, g.bg3d, g.zp, g.tx, g.ty, ncu),
+    g.nblk = std::min(stage_batch_blocks(b->f32, g.mhd, g.bg3d, g.zp, g.vw, g.tx, g.ty, ncu),
                       (start.back() + 7) / 8 * 8);
     if (g.zp) {
       std::vector<int> zs(g.members.size() + 1, 0);
@@ -1181,7 +1195,8 @@ static int batch_time_stage_t(sparc_batch* b, int reps, double* ms) {
       const int nm = (int)g.members.size();
       if (g.zp) e = launch_zpass_batch<T>(args, g.zitem_start, nm, g.znblk, b->st);
       if (e == hipSuccess)
-        e = launch_stage_batch<T>(g.mhd, g.bg3d, g.zp, g.tx, g.ty, args, g.item_start, nm, g.nblk, b->st);
+        e = launch_stage_batch<T>(g.mhd, g.bg3d, g.zp, g.vw, g.tx, g.ty, args, g.item_start, nm, g.nblk,
+                                  b->st);
     }
   }
   if (e == hipSuccess) e = hipEventRecord(e1, b->st);

@@ -18,17 +18,36 @@ namespace sparc {
 // ---------------------------------------------------------------------------
 // Filters
 // ---------------------------------------------------------------------------
+// A block filters one segment of one x row (j, k) of one field.  Rows are
+// numbered so that the rows a point's filter reads (same x, the filtered axis
+// +-G) are consecutive (y filter: j fastest; z filter: k fastest), and the
+// block ids are remapped so each XCD takes a contiguous range of rows: the
+// 2G+1 reads of a point then hit the XCD's L2 and HBM sees about one read per
+// value.  The row decode is a handful of uniform 32-bit divisions per block.
 template <typename T, int AXIS, int G>
 __global__ __launch_bounds__(256) void k_filter(const T* __restrict__ in, T* __restrict__ out,
-                                                int nx, int ny, int nz, long fs, int periodic,
-                                                T s, int bare) {
-  const long N = (long)nx * ny * nz;
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= N) return;
-  const int f = blockIdx.y;
-  const int i = (int)(idx % nx);
-  const int j = (int)((idx / nx) % ny);
-  const int k = (int)(idx / ((long)nx * ny));
+                                                int nx, int ny, int nz, long fs, int nseg,
+                                                int periodic, T s, int bare) {
+  const int nblk = gridDim.x, b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
+  const int lg = xcd * base + (xcd < rem ? xcd : rem) + loc;
+  const int seg = lg % nseg;
+  int row = lg / nseg, j, k;
+  if (AXIS == 2) {
+    k = row % nz;
+    row /= nz;
+    j = row % ny;
+    row /= ny;
+  } else {
+    j = row % ny;
+    row /= ny;
+    k = row % nz;
+    row /= nz;
+  }
+  const int f = row;
+  const int i = seg * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= nx) return;
+  const long idx = (long)i + (long)nx * (j + (long)ny * k);
   const int n = AXIS == 0 ? nx : AXIS == 1 ? ny : nz;
   const int pos = AXIS == 0 ? i : AXIS == 1 ? j : k;
   const long stride = AXIS == 0 ? 1 : AXIS == 1 ? (long)nx : (long)nx * ny;
@@ -64,10 +83,15 @@ __global__ __launch_bounds__(256) void k_filter(const T* __restrict__ in, T* __r
 template <typename T>
 hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
                          int nfield, int periodic, T s, int bare, hipStream_t st) {
-  const long N = (long)nx * ny * nz;
-  dim3 grid((unsigned)((N + 255) / 256), (unsigned)nfield);
+  // one block per x-row segment: 256 threads, or the row rounded up to
+  // whole waves when it is shorter
+  const int nt = nx >= 256 ? 256 : (nx + 63) / 64 * 64;
+  const int nseg = (nx + nt - 1) / nt;
+  const long nb = (long)nseg * ny * nz * nfield;
+  if (nb >= (1L << 31)) return hipErrorInvalidValue;
+  dim3 grid((unsigned)nb);
 #define SPARC_F(AX, GG) \
-  hipLaunchKernelGGL((k_filter<T, AX, GG>), grid, dim3(256), 0, st, in, out, nx, ny, nz, fs, periodic, s, bare)
+  hipLaunchKernelGGL((k_filter<T, AX, GG>), grid, dim3(nt), 0, st, in, out, nx, ny, nz, fs, nseg, periodic, s, bare)
   if (g == 5) {
     if (axis == 0) SPARC_F(0, 5);
     else if (axis == 1) SPARC_F(1, 5);

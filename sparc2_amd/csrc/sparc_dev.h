@@ -184,33 +184,89 @@ template <typename T> struct Buf {
   }
 };
 
-// CPML update at one point of one face (RHS.f90:52-55 per derivative
-// family): the psi RHS comes from the raw derivative, then the derivative is
-// replaced by d/kappa + psi; the next psi stage value (step.f90:34-36) or, on
-// the last stage, the advanced base (step.f90:59-61) is written in place.
-// v[i] is the derivative of psi family fam[i].  All loads of the face point
-// (a, b, 1/kappa once, then every family's base and stage value) are issued
-// before any store, so they share one memory wait.  FR is a Face<T> in any
-// address space.
-template <typename T, int NF, typename FR>
-__device__ __forceinline__ void pml_face(FR& F, uint32_t fp, T (&v)[NF], const int (&fam)[NF],
+// ---------------------------------------------------------------------------
+// Lane values.  The FP32 stage kernels on grids with even nx give each lane
+// two adjacent x points: one 8-byte load or store moves both, so a wave
+// moves as many bytes per memory instruction as the FP64 kernels do.  F2 is
+// that pair; its arithmetic is elementwise, in the operation order of the
+// scalar code, so a two-point lane computes what two one-point lanes would.
+// ---------------------------------------------------------------------------
+struct F2 {
+  float x, y;
+  F2() = default;
+  __host__ __device__ constexpr F2(float s) : x(s), y(s) {}
+  __host__ __device__ constexpr F2(float a, float b) : x(a), y(b) {}
+};
+__device__ __forceinline__ F2 operator+(F2 a, F2 b) { return F2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ F2 operator-(F2 a, F2 b) { return F2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ F2 operator*(F2 a, F2 b) { return F2(a.x * b.x, a.y * b.y); }
+__device__ __forceinline__ F2 operator-(F2 a) { return F2(-a.x, -a.y); }
+__device__ __forceinline__ F2& operator+=(F2& a, F2 b) { return a = a + b; }
+
+// V = the value one lane holds: T (one point) or F2 (two points, FP32 only)
+template <typename T, int VW> struct Lane { using V = T; };
+template <> struct Lane<float, 2> { using V = F2; };
+
+// point c of a lane value
+template <typename T> __device__ __forceinline__ T lane_pt(T v, int) { return v; }
+__device__ __forceinline__ float lane_pt(F2 v, int c) { return c ? v.y : v.x; }
+template <typename T> __device__ __forceinline__ void set_lane_pt(T& v, int, T s) { v = s; }
+__device__ __forceinline__ void set_lane_pt(F2& v, int c, float s) {
+  if (c) v.y = s;
+  else v.x = s;
+}
+
+// Lane-value load / store through a buffer view of T elements: byte offset
+// bo (Buf::kOut reads 0) or element index e of the lane's first point.
+template <typename V, typename T>
+__device__ __forceinline__ V ldv(const Buf<T>& b, uint32_t bo) {
+  if constexpr (sizeof(V) == sizeof(T)) {
+    return b.ldb(bo);
+  } else {
+    static_assert(sizeof(V) == 8 && sizeof(T) == 4, "two FP32 points per lane");
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)bo, 0, 0));
+  }
+}
+template <typename V, typename T>
+__device__ __forceinline__ void stv(const Buf<T>& b, uint32_t e, V v) {
+  if constexpr (sizeof(V) == sizeof(T)) {
+    b.st(e, v);
+  } else {
+    using R = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(R, v), b.r, (int)(e * 4u), 0, 0);
+  }
+}
+
+// CPML update at one point (or one lane's point pair) of one face
+// (RHS.f90:52-55 per derivative family): the psi RHS comes from the raw
+// derivative, then the derivative is replaced by d/kappa + psi; the next psi
+// stage value (step.f90:34-36) or, on the last stage, the advanced base
+// (step.f90:59-61) is written in place.  v[i] is the derivative of psi
+// family fam[i]; fp the face element of the lane's first point.  All loads
+// of the face point (a, b, 1/kappa once, then every family's base and stage
+// value) are issued before any store, so they share one memory wait.  FR is
+// a Face<T> in any address space.
+template <typename T, typename V, int NF, typename FR>
+__device__ __forceinline__ void pml_face(FR& F, uint32_t fp, V (&v)[NF], const int (&fam)[NF],
                                          bool first, bool last, T beta_next, T dt) {
   const uint32_t n = (uint32_t)F.fsize;
-  const T ca = Buf<T>(F.a, n)[fp], cb = Buf<T>(F.b, n)[fp], ck = Buf<T>(F.kinv, n)[fp];
-  T base[NF], ps[NF];
+  const uint32_t bo = fp * (uint32_t)sizeof(T);
+  const V ca = ldv<V>(Buf<T>(F.a, n), bo), cb = ldv<V>(Buf<T>(F.b, n), bo),
+          ck = ldv<V>(Buf<T>(F.kinv, n), bo);
+  V base[NF], ps[NF];
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
-    base[i] = Buf<T>(F.psi + (size_t)fam[i] * n, n)[fp];
-    ps[i] = first ? base[i] : Buf<T>(F.pst + (size_t)fam[i] * n, n)[fp];
+    base[i] = ldv<V>(Buf<T>(F.psi + (size_t)fam[i] * n, n), bo);
+    ps[i] = first ? base[i] : ldv<V>(Buf<T>(F.pst + (size_t)fam[i] * n, n), bo);
   }
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
-    const T kn = ca * v[i] + cb * ps[i];
+    const V kn = ca * v[i] + cb * ps[i];
     v[i] = v[i] * ck + ps[i];
     if (last)
-      Buf<T>(F.psi + (size_t)fam[i] * n, n).st(fp, base[i] + dt * kn);
+      stv(Buf<T>(F.psi + (size_t)fam[i] * n, n), fp, base[i] + dt * kn);
     else
-      Buf<T>(F.pst + (size_t)fam[i] * n, n).st(fp, base[i] + beta_next * kn);
+      stv(Buf<T>(F.pst + (size_t)fam[i] * n, n), fp, base[i] + beta_next * kn);
   }
 }
 

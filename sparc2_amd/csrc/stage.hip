@@ -26,8 +26,11 @@ namespace sparc {
 // so it runs 2 waves per SIMD; the MHD xy kernel (z derivatives from the z
 // pass) is held to 3.  The quiet kernel keeps the next plane's values in
 // registers, which fits 3 waves only in FP32 without 3-D backgrounds.
-template <typename T, bool MHD, bool BG3D, bool ZP>
+// A lane of two FP32 points holds as many register bytes as an FP64 lane, so
+// it takes the FP64 occupancy.
+template <typename T, bool MHD, bool BG3D, bool ZP, int VW>
 constexpr int stage_waves() {
+  if (VW == 2) return 2;
   if (MHD) return (ZP && sizeof(T) == 4) ? 3 : 2;
   // quiet with the one-plane-ahead point values: FP64, or FP32 with 3-D
   // backgrounds, need more registers than 3 waves per SIMD allow
@@ -37,12 +40,17 @@ constexpr int stage_waves() {
 // One work item: tile `tile` of z chunk `chunk` (logical = chunk*nblk_xy + tile)
 // of the run whose arguments Ap points to.  ZP (MHD only): the z derivatives,
 // CPML-corrected on the z faces, come from the z pass (Ap->dz) instead of the
-// register queue.
-template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
+// register queue.  VW: x points per lane (2: FP32 on even nx, F2 values); the
+// tile is TX lanes = VW*TX points wide.
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP, int VW>
 __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   static_assert(MHD || !ZP, "the z pass serves the MHD stage only");
+  static_assert(VW == 1 || (sizeof(T) == 4 && (ZP || !MHD)), "two-point lanes: FP32 quiet or MHD xy");
+  using V = typename Lane<T, VW>::V;
   constexpr int NT = TX * TY;
-  constexpr int LX = TX + 10, LY = TY + 10;
+  // x halo in lane columns: 5 points, rounded up to whole lanes
+  constexpr int HX = VW == 1 ? 5 : 3;
+  constexpr int LX = TX + 2 * HX, LY = TY + 10;
   constexpr int NR = TY * LX + 10 * TX;  // cross-shaped staging region
   constexpr int NS = (NR + NT - 1) / NT;
   constexpr int NQ = ZP ? 1 : (MHD ? 6 : 2);
@@ -61,7 +69,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   constexpr int NPRE = MHD ? 10 : 3;  // raw values staged per cell
   // one spare cell per plane takes the writes of slots past the region
   constexpr int LP = LY * LX + 1;
-  __shared__ T LB[NB][NL][LP];
+  __shared__ V LB[NB][NL][LP];
   __shared__ double rs[3][NT / 64 + 1];
 
   const int tid = threadIdx.x;
@@ -73,10 +81,11 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
 
   const int nblk_xy = Ap->nblk_xy, ntx = Ap->ntx, kc = Ap->kc;
   const int tile = logical % nblk_xy, chunk = logical / nblk_xy;
-  const int i0 = (tile % ntx) * TX, j0 = (tile / ntx) * TY;
+  // i0, i: x point indices (the tile's first point, the lane's first point)
+  const int i0 = (tile % ntx) * TX * VW, j0 = (tile / ntx) * TY;
   const int k0 = chunk * kc;
   const int k1 = min(k0 + kc, nz);
-  const int i = i0 + tx, j = j0 + ty;
+  const int i = i0 + VW * tx, j = j0 + ty;
   const bool active = (i < nx) && (j < ny);
   const uint32_t ij = active ? (uint32_t)(i + nx * j) : 0u;
 
@@ -100,39 +109,41 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       const int r2 = r - TY * LX;
       const int hr = r2 / TX;
       row = hr < 5 ? hr : hr + TY;
-      col = 5 + r2 % TX;
+      col = HX + r2 % TX;
     }
-    int gi = i0 - 5 + col, gj = j0 - 5 + row;
+    // gi: lane column (VW points from x = VW*gi)
+    const int nxl = nx / VW;
+    int gi = i0 / VW - HX + col, gj = j0 - 5 + row;
     bool ok = r < NR;
     if (periodic) {
-      gi = wrap(gi, nx);
+      gi = wrap(gi, nxl);
       gj = wrap(gj, ny);
     } else {
-      ok = ok && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+      ok = ok && gi >= 0 && gi < nxl && gj >= 0 && gj < ny;
     }
     s_ok[s] = ok;
-    s_off[s] = ok ? (uint32_t)(gi + nx * gj) : 0u;
+    s_off[s] = ok ? (uint32_t)(VW * gi + nx * gj) : 0u;
     s_lds[s] = r < NR ? row * LX + col : LY * LX;
   }
 
   // Array views, rebuilt from the argument pointer where they are used.
+  // g: element of the lane's first point; bo: byte offset (Buf::kOut reads 0)
   const T sxs = Ap->sxs, sys = Ap->sys;
-  auto UF = [&](int f, uint32_t g) -> T { return Buf<T>(Ap->u[f], fs)[g]; };
-  auto B3 = [&](const T* p, uint32_t g) -> T { return Buf<T>(p, fs)[g]; };
-  // by byte offset (Buf::kOut reads 0)
-  auto UFb = [&](int f, uint32_t bo) -> T { return Buf<T>(Ap->u[f], fs).ldb(bo); };
-  auto B3b = [&](const T* p, uint32_t bo) -> T { return Buf<T>(p, fs).ldb(bo); };
   constexpr uint32_t W = sizeof(T);
+  auto UF = [&](int f, uint32_t g) -> V { return ldv<V>(Buf<T>(Ap->u[f], fs), g * W); };
+  auto B3 = [&](const T* p, uint32_t g) -> V { return ldv<V>(Buf<T>(p, fs), g * W); };
+  auto UFb = [&](int f, uint32_t bo) -> V { return ldv<V>(Buf<T>(Ap->u[f], fs), bo); };
+  auto B3b = [&](const T* p, uint32_t bo) -> V { return ldv<V>(Buf<T>(p, fs), bo); };
   auto slot_bo = [&](int s, uint32_t pk) -> uint32_t {
     return s_ok[s] ? (pk + s_off[s]) * W : Buf<T>::kOut;
   };
-  auto flux1 = [&](uint32_t g) -> T { return -UF(VZ, g) * B3(Ap->boy, g) + UF(VY, g) * B3(Ap->boz, g); };
-  auto flux2 = [&](uint32_t g) -> T { return -UF(VX, g) * B3(Ap->boz, g) + UF(VZ, g) * B3(Ap->box, g); };
+  auto flux1 = [&](uint32_t g) -> V { return -UF(VZ, g) * B3(Ap->boy, g) + UF(VY, g) * B3(Ap->boz, g); };
+  auto flux2 = [&](uint32_t g) -> V { return -UF(VX, g) * B3(Ap->boz, g) + UF(VZ, g) * B3(Ap->box, g); };
 
-  T(*L)[LP] = LB[0];  // the plane buffer in use
+  V(*L)[LP] = LB[0];  // the plane buffer in use
   // SPF: staging values of the next plane, held in registers (MHD: v, p,
   // b and B0; the fluxes are formed when the plane is written to LDS)
-  T pre[NS][NPRE];
+  V pre[NS][NPRE];
   auto pre_load = [&](int k) {
     const uint32_t pk = (uint32_t)k * nxy;
 #pragma unroll
@@ -156,12 +167,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const int c = s_lds[s];
-      const T vx = pre[s][0], vy = pre[s][1];
+      const V vx = pre[s][0], vy = pre[s][1];
       L[0][c] = vx;
       L[1][c] = vy;
       L[2][c] = pre[s][2];
       if constexpr (MHD) {
-        const T vz = pre[s][3], b0x = pre[s][7], b0y = pre[s][8], b0z = pre[s][9];
+        const V vz = pre[s][3], b0x = pre[s][7], b0y = pre[s][8], b0z = pre[s][9];
         L[3][c] = -vy * b0x + vx * b0y;  // F3 = -vy box + vx boy
         L[4][c] = -vx * b0z + vz * b0x;  // F2
         L[5][c] = -vz * b0y + vy * b0z;  // F1
@@ -177,13 +188,13 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     for (int s = 0; s < NS; ++s) {
       const uint32_t bo = slot_bo(s, pk);
       const int c = s_lds[s];
-      const T vx = UFb(VX, bo), vy = UFb(VY, bo), p = UFb(P, bo);
+      const V vx = UFb(VX, bo), vy = UFb(VY, bo), p = UFb(P, bo);
       L[0][c] = vx;
       L[1][c] = vy;
       L[2][c] = p;
       if constexpr (MHD) {
-        const T vz = UFb(VZ, bo), bx = UFb(BX, bo), by = UFb(BY, bo), bz = UFb(BZ, bo);
-        const T b0x = B3b(Ap->box, bo), b0y = B3b(Ap->boy, bo), b0z = B3b(Ap->boz, bo);
+        const V vz = UFb(VZ, bo), bx = UFb(BX, bo), by = UFb(BY, bo), bz = UFb(BZ, bo);
+        const V b0x = B3b(Ap->box, bo), b0y = B3b(Ap->boy, bo), b0z = B3b(Ap->boz, bo);
         L[3][c] = -vy * b0x + vx * b0y;  // F3 = -vy box + vx boy
         L[4][c] = -vx * b0z + vz * b0x;  // F2
         L[5][c] = -vz * b0y + vy * b0z;  // F1
@@ -193,29 +204,60 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       }
     }
   };
-  // 11-point derivative along x / y from LDS (derivatives.f90:38-44,66-75,92-101)
-  auto dx_lds = [&](int f) -> T {
-    const T* row = &L[f][(ty + 5) * LX];
-    T acc = 0;
-    if (!periodic && i < 5) {
+  // 11-point derivative along x / y from LDS (derivatives.f90:38-44,66-75,92-101).
+  // x: the centred row for every point of the lane, then the one-sided rows
+  // for points within 5 of a non-periodic edge, reading the LDS row as
+  // points (point x sits at index x - i0 + VW*HX).
+  auto dx_lds = [&](int f) -> V {
+    const V* row = &L[f][(ty + 5) * LX];
+    V out;
+    if constexpr (VW == 1) {
+      T acc = 0;
+      const int c0 = tx + HX;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) acc += row[min(m - i0 + 5, LX - 1)] * Coef<T>::b(i, m);
-      return sxs * acc;
+      for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (row[c0 + m] - row[c0 - m]);
+      out = sxs * acc;
+    } else {
+      T e[14];  // points i-6 .. i+7
+#pragma unroll
+      for (int q2 = 0; q2 < 7; ++q2) {
+        const V w = row[tx + HX - 3 + q2];
+        e[2 * q2] = w.x;
+        e[2 * q2 + 1] = w.y;
+      }
+      T a0 = 0, a1 = 0;
+#pragma unroll
+      for (int m = 1; m <= 5; ++m) {
+        a0 += Coef<T>::c(m - 1) * (e[6 + m] - e[6 - m]);
+        a1 += Coef<T>::c(m - 1) * (e[7 + m] - e[7 - m]);
+      }
+      out = V(sxs * a0, sxs * a1);
     }
-    if (!periodic && i >= nx - 5) {
-      const int r = nx - 1 - i;
+    if (!periodic && (i < 5 || i + VW - 1 >= nx - 5)) {
+      constexpr int XO = VW * HX, RW = VW * LX;
+      const T* rp = (const T*)row;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) acc += row[min(max(nx - 1 - m - i0 + 5, 0), LX - 1)] * Coef<T>::b(r, m);
-      return -sxs * acc;
+      for (int c = 0; c < VW; ++c) {
+        const int p = i + c;
+        T acc = 0;
+        if (p < 5) {
+#pragma unroll
+          for (int m = 0; m < 11; ++m) acc += rp[min(m - i0 + XO, RW - 1)] * Coef<T>::b(p, m);
+          set_lane_pt(out, c, sxs * acc);
+        } else if (p >= nx - 5) {
+          const int r = nx - 1 - p;
+#pragma unroll
+          for (int m = 0; m < 11; ++m)
+            acc += rp[min(max(nx - 1 - m - i0 + XO, 0), RW - 1)] * Coef<T>::b(r, m);
+          set_lane_pt(out, c, -sxs * acc);
+        }
+      }
     }
-    const int c0 = tx + 5;
-#pragma unroll
-    for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (row[c0 + m] - row[c0 - m]);
-    return sxs * acc;
+    return out;
   };
-  auto dy_lds = [&](int f) -> T {
-    const T* col = &L[f][tx + 5];
-    T acc = 0;
+  auto dy_lds = [&](int f) -> V {
+    const V* col = &L[f][tx + HX];
+    V acc = 0;
     if (!periodic && j < 5) {
 #pragma unroll
       for (int m = 0; m < 11; ++m) acc += col[min(max(m - j0 + 5, 0), LY - 1) * LX] * Coef<T>::b(j, m);
@@ -235,10 +277,10 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
 
   // z register queue: slot m holds plane k-5+m of each z-differentiated
   // field.  Quiet: vz, p.  MHD: vz, p, bx, by, flux1, flux2.  (Unused with ZP.)
-  T q[NQ][11];
-  T nq[NQ];  // next plane, loaded one iteration ahead
+  V q[NQ][11];
+  V nq[NQ];  // next plane, loaded one iteration ahead
   // planes outside 0..nz-1 read as 0: the zero ghosts below the bottom
-  auto fetch = [&](int kk, T* dst) {
+  auto fetch = [&](int kk, V* dst) {
     if constexpr (ZP) return;
     const uint32_t bo = (active && kk >= 0 && kk < nz) ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
     dst[0] = UFb(VZ, bo);
@@ -246,7 +288,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     if constexpr (MHD) {
       dst[2] = UFb(BX, bo);
       dst[3] = UFb(BY, bo);
-      const T vx = UFb(VX, bo), vy = UFb(VY, bo), vz = dst[0];
+      const V vx = UFb(VX, bo), vy = UFb(VY, bo), vz = dst[0];
       dst[4] = -vz * B3b(Ap->boy, bo) + vy * B3b(Ap->boz, bo);
       dst[5] = -vx * B3b(Ap->boz, bo) + vz * B3b(Ap->box, bo);
     }
@@ -254,7 +296,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   if constexpr (!ZP) {
 #pragma unroll
     for (int m = 0; m < 10; ++m) {
-      T t[NQ];
+      V t[NQ];
       fetch(k0 - 5 + m, t);
 #pragma unroll
       for (int f = 0; f < NQ; ++f) q[f][m] = t[f];
@@ -265,9 +307,9 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   double ke = 0.0, mr = 1.0e300, mp = 1.0e300;
   const T dt = Ap->dt, bnext = Ap->beta_next;
   // background at the point: 3-D array, or the z profile (uniform, scalar load)
-  auto BG = [&](const T* p, uint32_t g, int k) -> T {
-    if constexpr (BG3D) return Buf<T>(p, fs)[g];
-    else return ((CPtr<T>)p)[k];
+  auto BG = [&](const T* p, uint32_t g, int k) -> V {
+    if constexpr (BG3D) return B3(p, g);
+    else return V(((CPtr<T>)p)[k]);
   };
 
   // Quiet: the point values of a plane -- rho, the RK base a, the 3-D
@@ -283,13 +325,13 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   constexpr int NPP = (PPF && BG3D) ? 1 : 0;             // p0, last stage
   constexpr int IC = 1 + NV + NPB, IP = IC + NPC;         // slots of comp and p0
   constexpr int NP = PPF ? IP + NPP : 1;
-  T pc[NP], pn[NP];
-  auto point_load = [&](int kk, T* dst) {
+  V pc[NP], pn[NP];
+  auto point_load = [&](int kk, V* dst) {
     if constexpr (PPF) {
       const uint32_t bo = active ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
       dst[0] = UFb(RHO, bo);
 #pragma unroll
-      for (int f = 0; f < NV; ++f) dst[1 + f] = Buf<T>(Ap->a[f], fs).ldb(bo);
+      for (int f = 0; f < NV; ++f) dst[1 + f] = ldv<V>(Buf<T>(Ap->a[f], fs), bo);
       if constexpr (NPB > 0) {
         dst[1 + NV + 0] = B3b(Ap->rho0, bo);
         dst[1 + NV + 1] = B3b(Ap->rhoinv, bo);
@@ -301,7 +343,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       if (last) {
         if constexpr (NPC > 0) {
 #pragma unroll
-          for (int f = 0; f < NV; ++f) dst[IC + f] = Buf<T>(Ap->comp[f], fs).ldb(bo);
+          for (int f = 0; f < NV; ++f) dst[IC + f] = ldv<V>(Buf<T>(Ap->comp[f], fs), bo);
         }
         if constexpr (NPP > 0) dst[IP] = B3b(Ap->p0, bo);
       }
@@ -314,16 +356,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   constexpr bool MPX = MHD && ZP;
   constexpr int MP_A = 8, MP_BG = 16, MP_P0 = 33, MP_C = 34;
   constexpr int NPM = MPX ? MP_C + (sizeof(T) == 4 ? 8 : 0) : 1;
-  T pm[NPM];
+  V pm[NPM];
   auto mpoint_load = [&](int kk) {
     if constexpr (MPX) {
       const uint32_t bo = active ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
       pm[0] = UFb(RHO, bo);
       pm[1] = UFb(VZ, bo);
 #pragma unroll
-      for (int f = 0; f < 6; ++f) pm[2 + f] = Buf<T>(Ap->dz[f], fs).ldb(bo);
+      for (int f = 0; f < 6; ++f) pm[2 + f] = ldv<V>(Buf<T>(Ap->dz[f], fs), bo);
 #pragma unroll
-      for (int f = 0; f < 8; ++f) pm[MP_A + f] = Buf<T>(Ap->a[f], fs).ldb(bo);
+      for (int f = 0; f < 8; ++f) pm[MP_A + f] = ldv<V>(Buf<T>(Ap->a[f], fs), bo);
       const T* const bgp[17] = {Ap->rho0, Ap->rhoinv, Ap->sponge, Ap->red, Ap->box, Ap->boy,
                                 Ap->boz, Ap->gr0x, Ap->gr0y, Ap->gr0z, Ap->c2rho0, Ap->gp0x,
                                 Ap->gp0y, Ap->gp0z, Ap->cbox, Ap->cboy, Ap->cboz};
@@ -333,13 +375,13 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         pm[MP_P0] = B3b(Ap->p0, bo);
         if constexpr (sizeof(T) == 4) {
 #pragma unroll
-          for (int f = 0; f < 8; ++f) pm[MP_C + f] = Buf<T>(Ap->comp[f], fs).ldb(bo);
+          for (int f = 0; f < 8; ++f) pm[MP_C + f] = ldv<V>(Buf<T>(Ap->comp[f], fs), bo);
         }
       }
     }
   };
   // background slot at the point in pm (MHD xy kernel)
-  auto PMB = [&](int q2) -> T { return pm[MP_BG + q2]; };
+  auto PMB = [&](int q2) -> V { return pm[MP_BG + q2]; };
 
   if constexpr (SPF) pre_load(k0);
   if constexpr (PPF) point_load(k0, pn);
@@ -364,7 +406,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       stage_plane(k);
     }
     // ---- z derivatives (derivatives.f90:481-487,509-518,536-545) ----
-    T dz[NQ];
+    V dz[NQ];
     if constexpr (!ZP) {
 #pragma unroll
       for (int f = 0; f < NQ; ++f) q[f][10] = nq[f];
@@ -374,7 +416,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         if (k < nz - 5) {
 #pragma unroll
           for (int f = 0; f < NQ; ++f) {
-            T acc = 0;
+            V acc = 0;
 #pragma unroll
             for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (q[f][5 + m] - q[f][5 - m]);
             dz[f] = st * acc;
@@ -382,7 +424,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         } else {
           // top block: mirrored one-sided rows over planes nz-1 .. nz-11
           const int r = nz - 1 - k;
-          T acc[NQ];
+          V acc[NQ];
 #pragma unroll
           for (int f = 0; f < NQ; ++f) acc[f] = 0;
           for (int m = 0; m < 11; ++m) {
@@ -408,16 +450,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     if (active) {
       const uint32_t g = ij + (uint32_t)k * nxy;
       // xy derivatives and centre values from LDS
-      T dvxdx = dx_lds(0), dvydy = dy_lds(1), gpx = dx_lds(2), gpy = dy_lds(2);
-      const int cc = (ty + 5) * LX + tx + 5;
-      const T vx = L[0][cc], vy = L[1][cc];
-      const T rho = PPF ? pc[0] : (MPX ? pm[0] : UF(RHO, g));
-      const T vz = ZP ? (MPX ? pm[1] : UF(VZ, g)) : q[0][5], p = ZP ? L[2][cc] : q[1][5];
+      V dvxdx = dx_lds(0), dvydy = dy_lds(1), gpx = dx_lds(2), gpy = dy_lds(2);
+      const int cc = (ty + 5) * LX + tx + HX;
+      const V vx = L[0][cc], vy = L[1][cc];
+      const V rho = PPF ? pc[0] : (MPX ? pm[0] : UF(RHO, g));
+      const V vz = ZP ? (MPX ? pm[1] : UF(VZ, g)) : q[0][5], p = ZP ? L[2][cc] : q[1][5];
       // z pass output, in CPML family order: vz, p, bx, by, F2, F1
-      auto DZ = [&](int f) -> T { return MPX ? pm[2 + f] : Buf<T>(Ap->dz[f], fs)[g]; };
-      T dvzdz = ZP ? DZ(0) : dz[0], gpz = ZP ? DZ(1) : dz[1];
-      T dxf3 = 0, dyf3 = 0, dxf2 = 0, dyf1 = 0, dxby = 0, dxbz = 0, dybz = 0, dybx = 0, bz = 0;
-      T dzbx = 0, dzby = 0, dzf1 = 0, dzf2 = 0;
+      auto DZ = [&](int f) -> V { return MPX ? pm[2 + f] : ldv<V>(Buf<T>(Ap->dz[f], fs), g * W); };
+      V dvzdz = ZP ? DZ(0) : dz[0], gpz = ZP ? DZ(1) : dz[1];
+      V dxf3 = 0, dyf3 = 0, dxf2 = 0, dyf1 = 0, dxby = 0, dxbz = 0, dybz = 0, dybx = 0, bz = 0;
+      V dzbx = 0, dzby = 0, dzf1 = 0, dzf2 = 0;
       if constexpr (MHD) {
         dxf3 = dx_lds(3);
         dyf3 = dy_lds(3);
@@ -450,33 +492,46 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         if (!F.active || kp < 0 || kp >= F.n) continue;
         const uint32_t fp = ij + nxy * (uint32_t)kp;
         if constexpr (MHD) {
-          T v[6] = {gpz, dvzdz, dzf2, dzf1, dzbx, dzby};
+          V v[6] = {gpz, dvzdz, dzf2, dzf1, dzbx, dzby};
           constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
           pml_face(F, fp, v, fam, first, last, bnext, dt);
           gpz = v[0], dvzdz = v[1], dzf2 = v[2], dzf1 = v[3], dzbx = v[4], dzby = v[5];
         } else {
-          T v[2] = {gpz, dvzdz};
+          V v[2] = {gpz, dvzdz};
           constexpr int fam[2] = {1, 0};
           pml_face(F, fp, v, fam, first, last, bnext, dt);
           gpz = v[0], dvzdz = v[1];
         }
       }
+      // x faces point by point: the points of a lane can straddle the face
+      // edge, and an x face row (n points) need not start on a lane boundary
 #pragma unroll
       for (int fx = XBOT; fx <= XTOP; ++fx) {
         const auto& F = Ap->face[fx];
-        const int ip = i - F.off;
-        if (!F.active || ip < 0 || ip >= F.n) continue;
-        const uint32_t fp = (uint32_t)(ip + F.n * (j + ny * k));
-        if constexpr (MHD) {
-          T v[6] = {gpx, dvxdx, dxf3, dxf2, dxby, dxbz};
-          constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
-          pml_face(F, fp, v, fam, first, last, bnext, dt);
-          gpx = v[0], dvxdx = v[1], dxf3 = v[2], dxf2 = v[3], dxby = v[4], dxbz = v[5];
-        } else {
-          T v[2] = {gpx, dvxdx};
-          constexpr int fam[2] = {1, 0};
-          pml_face(F, fp, v, fam, first, last, bnext, dt);
-          gpx = v[0], dvxdx = v[1];
+        if (!F.active) continue;
+#pragma unroll
+        for (int c = 0; c < VW; ++c) {
+          const int ip = i + c - F.off;
+          if (ip < 0 || ip >= F.n) continue;
+          const uint32_t fp = (uint32_t)(ip + F.n * (j + ny * k));
+          if constexpr (MHD) {
+            T v[6] = {lane_pt(gpx, c),  lane_pt(dvxdx, c), lane_pt(dxf3, c),
+                      lane_pt(dxf2, c), lane_pt(dxby, c),  lane_pt(dxbz, c)};
+            constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
+            pml_face(F, fp, v, fam, first, last, bnext, dt);
+            set_lane_pt(gpx, c, v[0]);
+            set_lane_pt(dvxdx, c, v[1]);
+            set_lane_pt(dxf3, c, v[2]);
+            set_lane_pt(dxf2, c, v[3]);
+            set_lane_pt(dxby, c, v[4]);
+            set_lane_pt(dxbz, c, v[5]);
+          } else {
+            T v[2] = {lane_pt(gpx, c), lane_pt(dvxdx, c)};
+            constexpr int fam[2] = {1, 0};
+            pml_face(F, fp, v, fam, first, last, bnext, dt);
+            set_lane_pt(gpx, c, v[0]);
+            set_lane_pt(dvxdx, c, v[1]);
+          }
         }
       }
 #pragma unroll
@@ -486,12 +541,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         if (!F.active || jp < 0 || jp >= F.n) continue;
         const uint32_t fp = (uint32_t)(i + nx * (jp + F.n * k));
         if constexpr (MHD) {
-          T v[6] = {gpy, dvydy, dyf3, dyf1, dybx, dybz};
+          V v[6] = {gpy, dvydy, dyf3, dyf1, dybx, dybz};
           constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
           pml_face(F, fp, v, fam, first, last, bnext, dt);
           gpy = v[0], dvydy = v[1], dyf3 = v[2], dyf1 = v[3], dybx = v[4], dybz = v[5];
         } else {
-          T v[2] = {gpy, dvydy};
+          V v[2] = {gpy, dvydy};
           constexpr int fam[2] = {1, 0};
           pml_face(F, fp, v, fam, first, last, bnext, dt);
           gpy = v[0], dvydy = v[1];
@@ -500,14 +555,14 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
 
       // ---- pointwise RHS ----
       // background at the point (quiet 3-D backgrounds come from the prefetch)
-      auto BGP = [&](const T* p, int slot) -> T {
+      auto BGP = [&](const T* p, int slot) -> V {
         if constexpr (NPB > 0) return pc[1 + NV + slot];
         else if constexpr (MPX) return PMB(slot);  // rho0, rhoinv, sponge
         else return BG(p, g, k);
       };
-      const T rho0 = BGP(Ap->rho0, 0), rhoinv = BGP(Ap->rhoinv, 1), sp = BGP(Ap->sponge, 2);
-      const T div = dvxdx + dvydy + dvzdz;
-      T K[8];
+      const V rho0 = BGP(Ap->rho0, 0), rhoinv = BGP(Ap->rhoinv, 1), sp = BGP(Ap->sponge, 2);
+      const V div = dvxdx + dvydy + dvzdz;
+      V K[8];
       if constexpr (!MHD) {
         const T gk = ((CPtr<T>)Ap->g)[k];
         // RHS.f90:95-101,136,152
@@ -517,28 +572,28 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         K[VZ] = -rhoinv * (gpz + rho * gk) - vz * sp;
         K[P] = -BGP(Ap->c2rho0, 4) * div - vz * BGP(Ap->gp0z, 5) - sp * p;
       } else {
-        const T red = MPX ? PMB(3) : B3(Ap->red, g);
-        const T b0x = MPX ? PMB(4) : B3(Ap->box, g), b0y = MPX ? PMB(5) : B3(Ap->boy, g),
+        const V red = MPX ? PMB(3) : B3(Ap->red, g);
+        const V b0x = MPX ? PMB(4) : B3(Ap->box, g), b0y = MPX ? PMB(5) : B3(Ap->boy, g),
                 b0z = MPX ? PMB(6) : B3(Ap->boz, g);
-        const T bx = ZP ? L[8][cc] : q[2][5], by = ZP ? L[6][cc] : q[3][5];
+        const V bx = ZP ? L[8][cc] : q[2][5], by = ZP ? L[6][cc] : q[3][5];
         // RHS.f90:548-580
         K[BX] = red * (dyf3 - dzf2);
         K[BY] = red * (dzf1 - dxf3);
         K[BZ] = red * (dxf2 - dyf1);
-        const T cbx = dybz - dzby, cby = -dxbz + dzbx, cbz = dxby - dybx;
-        const T gr0x = MPX ? PMB(7) : B3(Ap->gr0x, g), gr0y = MPX ? PMB(8) : B3(Ap->gr0y, g),
+        const V cbx = dybz - dzby, cby = -dxbz + dzbx, cbz = dxby - dybx;
+        const V gr0x = MPX ? PMB(7) : B3(Ap->gr0x, g), gr0y = MPX ? PMB(8) : B3(Ap->gr0y, g),
                 gr0z = MPX ? PMB(9) : B3(Ap->gr0z, g), c2r = MPX ? PMB(10) : B3(Ap->c2rho0, g),
                 gp0x = MPX ? PMB(11) : B3(Ap->gp0x, g), gp0y = MPX ? PMB(12) : B3(Ap->gp0y, g),
                 gp0z = MPX ? PMB(13) : B3(Ap->gp0z, g);
         K[RHO] = -gr0x * vx - gr0y * vy - gr0z * vz - rho0 * div - sp * rho;
         K[P] = -c2r * div - vx * gp0x - vy * gp0y - vz * gp0z - sp * p;
         // CROSS(curl b, B0) and CROSS(curl B0, b), derivatives.f90:646-655
-        const T c1 = cby * b0z - cbz * b0y, c2 = b0x * cbz - cbx * b0z, c3 = cbx * b0y - cby * b0x;
-        const T cox = MPX ? PMB(14) : B3(Ap->cbox, g), coy = MPX ? PMB(15) : B3(Ap->cboy, g),
+        const V c1 = cby * b0z - cbz * b0y, c2 = b0x * cbz - cbx * b0z, c3 = cbx * b0y - cby * b0x;
+        const V cox = MPX ? PMB(14) : B3(Ap->cbox, g), coy = MPX ? PMB(15) : B3(Ap->cboy, g),
                 coz = MPX ? PMB(16) : B3(Ap->cboz, g);
-        const T e1 = coy * bz - coz * by;
-        const T e2 = bx * coz - cox * bz;
-        const T e3 = cox * by - coy * bx;
+        const V e1 = coy * bz - coz * by;
+        const V e2 = bx * coz - cox * bz;
+        const V e3 = cox * by - coy * bx;
         K[VX] = -rhoinv * (gpx - c1 * red) - sp * vx + rhoinv * e1 * red;
         K[VY] = -rhoinv * (gpy - c2 * red) - sp * vy + rhoinv * e2 * red;
         K[VZ] = -rhoinv * (gpz - c3 * red) - sp * vz + rhoinv * e3 * red - rhoinv * rho * ((CPtr<T>)Ap->g)[k];
@@ -547,59 +602,65 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       const int pdir = Ap->pulse_dir;
       if (pdir >= 1 && pdir <= 3) {
         const T amp = Ap->amp_tab ? Ap->amp_tab[*Ap->tstep] : Ap->src_amp;
-        const T s = amp * Buf<T>(Ap->sx, nx)[i] * Buf<T>(Ap->sy, ny)[j] * ((CPtr<T>)Ap->sz)[k];
+        const V s = amp * ldv<V>(Buf<T>(Ap->sx, nx), (uint32_t)i * W) * Buf<T>(Ap->sy, ny)[j] *
+                    ((CPtr<T>)Ap->sz)[k];
         if (pdir == 1) K[VX] += s;
         else if (pdir == 2) K[VY] += s;
         else K[VZ] += s;
       }
       // boundary-plane zeroing (RHS.f90:154-164,647-657)
-      const bool zero = (k == 0) || (k == nz - 1) ||
-                        (!periodic && (i == 0 || i == nx - 1 || j == 0 || j == ny - 1));
-      if (zero) {
+      const bool zero = (k == 0) || (k == nz - 1) || (!periodic && (j == 0 || j == ny - 1));
 #pragma unroll
-        for (int f = 0; f < NV; ++f) K[f] = 0;
+      for (int c = 0; c < VW; ++c) {
+        if (zero || (!periodic && (i + c == 0 || i + c == nx - 1))) {
+#pragma unroll
+          for (int f = 0; f < NV; ++f) set_lane_pt(K[f], c, T(0));
+        }
       }
       if (!last) {
         // next stage value, step.f90:33
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
           const Buf<T> uf(Ap->un[f], fs);
-          const T a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : Buf<T>(Ap->a[f], fs)[g]);
-          uf.st(g, a0 + bnext * K[f]);
+          const V a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : ldv<V>(Buf<T>(Ap->a[f], fs), g * W));
+          stv(uf, g, a0 + bnext * K[f]);
         }
       } else {
-        T an[NV];
+        V an[NV];
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
           const Buf<T> af(Ap->a[f], fs);
-          const T a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : af[g]);
+          const V a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : ldv<V>(af, g * W));
           if constexpr (sizeof(T) == 4) {
             // compensated update: a - c carries the bits a + dt K loses
             const Buf<T> cf(Ap->comp[f], fs);
-            T c0;
+            V c0;
             if constexpr (NPC > 0) c0 = pc[IC + f];
             else if constexpr (MPX) c0 = pm[MP_C + f];
-            else c0 = cf[g];
-            const T y = dt * K[f] - c0;
-            const T t = a0 + y;
-            cf.st(g, (t - a0) - y);
+            else c0 = ldv<V>(cf, g * W);
+            const V y = dt * K[f] - c0;
+            const V t = a0 + y;
+            stv(cf, g, (t - a0) - y);
             an[f] = t;
           } else {
             an[f] = a0 + dt * K[f];
           }
-          af.st(g, an[f]);
+          stv(af, g, an[f]);
         }
         // diagnostics: KE norm on the new state, minima on the stage-5 state
-        const double m = ((double)rho0 + (double)an[RHO]) *
-                         ((double)an[VX] * an[VX] + (double)an[VY] * an[VY] + (double)an[VZ] * an[VZ]) /
-                         2.0;
-        ke += m * m;
-        mr = fmin(mr, (double)rho + (double)rho0);
-        T p0v;
+        V p0v;
         if constexpr (NPP > 0) p0v = pc[IP];
         else if constexpr (MPX) p0v = pm[MP_P0];
         else p0v = BG(Ap->p0, g, k);
-        mp = fmin(mp, (double)p + (double)p0v);
+#pragma unroll
+        for (int c = 0; c < VW; ++c) {
+          const double r0 = lane_pt(rho0, c), ux = lane_pt(an[VX], c), uy = lane_pt(an[VY], c),
+                       uz = lane_pt(an[VZ], c);
+          const double m = (r0 + (double)lane_pt(an[RHO], c)) * (ux * ux + uy * uy + uz * uz) / 2.0;
+          ke += m * m;
+          mr = fmin(mr, (double)lane_pt(rho, c) + r0);
+          mp = fmin(mp, (double)lane_pt(p, c) + (double)lane_pt(p0v, c));
+        }
       }
     }
     // single buffer: the next plane's staging must wait for these reads
@@ -784,21 +845,21 @@ __global__ __launch_bounds__(ZP_NT, 2) void k_zpass_batch(const StageArgs<T>* __
 // Single run: one block per work item.  The hardware hands consecutive block
 // ids to the 8 XCDs in turn; remap so each XCD gets a contiguous run of tiles
 // and the halos the tiles share stay in that XCD's L2.
-template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
-__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP>())) void k_stage(const StageArgs<T> A) {
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP, int VW>
+__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP, VW>())) void k_stage(const StageArgs<T> A) {
   const int nblk = gridDim.x, b = blockIdx.x;
   const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
   const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
   // A sits at offset 0 of the kernel-argument segment
-  stage_item<T, MHD, BG3D, TX, TY, ZP>((CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr(), logical);
+  stage_item<T, MHD, BG3D, TX, TY, ZP, VW>((CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr(), logical);
 }
 
 // Many runs in one persistent launch: the work items of all runs form one
 // list (item_start[r] = first item of run r, item_start[nrun] = total).  Each
 // XCD takes a contiguous eighth of the list and its blocks stride through it.
 // The grid size must be a multiple of 8.
-template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
-__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP>())) void k_stage_batch(
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP, int VW>
+__global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP, VW>())) void k_stage_batch(
     const StageArgs<T>* __restrict__ runs, const int* __restrict__ item_start, int nrun) {
   const int total = item_start[nrun];
   const int xcd = blockIdx.x & 7, m = blockIdx.x >> 3, nb8 = gridDim.x >> 3;
@@ -807,62 +868,83 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP>())) void k_s
     int r = 0;
     while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
     // no kernel writes the argument array while the batch runs
-    stage_item<T, MHD, BG3D, TX, TY, ZP>((CArgs<T>)(runs + r), item - item_start[r]);
+    stage_item<T, MHD, BG3D, TX, TY, ZP, VW>((CArgs<T>)(runs + r), item - item_start[r]);
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers (explicit instantiations over the supported tile shapes).
+// Variants: MHD xy kernel (z pass), fused MHD, quiet with 3-D backgrounds,
+// quiet with z profiles; two-point lanes for FP32 quiet and MHD xy only.
 // ---------------------------------------------------------------------------
-template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP, int VW>
 static hipError_t launch_one(const StageArgs<T>& A, hipStream_t s) {
   const int nblk = A.nblk_xy * A.nchunk;
-  hipLaunchKernelGGL((k_stage<T, MHD, BG3D, TX, TY, ZP>), dim3(nblk), dim3(TX * TY), 0, s, A);
+  hipLaunchKernelGGL((k_stage<T, MHD, BG3D, TX, TY, ZP, VW>), dim3(nblk), dim3(TX * TY), 0, s, A);
   return hipGetLastError();
 }
 
-template <typename T, bool MHD, bool BG3D, bool ZP>
+template <typename T, bool MHD, bool BG3D, bool ZP, int VW>
 static hipError_t launch_tile(int tx, int ty, const StageArgs<T>& A, hipStream_t s) {
-  if (tx == 64 && ty == 4) return launch_one<T, MHD, BG3D, 64, 4, ZP>(A, s);
-  if (tx == 64 && ty == 8) return launch_one<T, MHD, BG3D, 64, 8, ZP>(A, s);
-  if (tx == 32 && ty == 8) return launch_one<T, MHD, BG3D, 32, 8, ZP>(A, s);
-  if (tx == 32 && ty == 16) return launch_one<T, MHD, BG3D, 32, 16, ZP>(A, s);
-  if (tx == 16 && ty == 16) return launch_one<T, MHD, BG3D, 16, 16, ZP>(A, s);
+  if (tx == 64 && ty == 4) return launch_one<T, MHD, BG3D, 64, 4, ZP, VW>(A, s);
+  if (tx == 64 && ty == 8) return launch_one<T, MHD, BG3D, 64, 8, ZP, VW>(A, s);
+  if (tx == 32 && ty == 8) return launch_one<T, MHD, BG3D, 32, 8, ZP, VW>(A, s);
+  if (tx == 32 && ty == 16) return launch_one<T, MHD, BG3D, 32, 16, ZP, VW>(A, s);
+  if (tx == 16 && ty == 16) return launch_one<T, MHD, BG3D, 16, 16, ZP, VW>(A, s);
   return hipErrorInvalidValue;
 }
 
-template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP>
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP, int VW>
 static hipError_t launch_batch_one(const StageArgs<T>* runs, const int* item_start, int nrun,
                                    int nblk, hipStream_t s) {
-  hipLaunchKernelGGL((k_stage_batch<T, MHD, BG3D, TX, TY, ZP>), dim3(nblk), dim3(TX * TY), 0, s,
+  hipLaunchKernelGGL((k_stage_batch<T, MHD, BG3D, TX, TY, ZP, VW>), dim3(nblk), dim3(TX * TY), 0, s,
                      runs, item_start, nrun);
   return hipGetLastError();
 }
 
-template <typename T, bool MHD, bool BG3D, bool ZP>
+template <typename T, bool MHD, bool BG3D, bool ZP, int VW>
 static hipError_t launch_batch_tile(int tx, int ty, const StageArgs<T>* runs, const int* item_start,
                                     int nrun, int nblk, hipStream_t s) {
-  if (tx == 64 && ty == 4) return launch_batch_one<T, MHD, BG3D, 64, 4, ZP>(runs, item_start, nrun, nblk, s);
-  if (tx == 64 && ty == 8) return launch_batch_one<T, MHD, BG3D, 64, 8, ZP>(runs, item_start, nrun, nblk, s);
-  if (tx == 32 && ty == 8) return launch_batch_one<T, MHD, BG3D, 32, 8, ZP>(runs, item_start, nrun, nblk, s);
-  if (tx == 32 && ty == 16) return launch_batch_one<T, MHD, BG3D, 32, 16, ZP>(runs, item_start, nrun, nblk, s);
-  if (tx == 16 && ty == 16) return launch_batch_one<T, MHD, BG3D, 16, 16, ZP>(runs, item_start, nrun, nblk, s);
+#define SPARC_BL(X, Y) \
+  if (tx == X && ty == Y) return launch_batch_one<T, MHD, BG3D, X, Y, ZP, VW>(runs, item_start, nrun, nblk, s);
+  SPARC_BL(64, 4)
+  SPARC_BL(64, 8)
+  SPARC_BL(32, 8)
+  SPARC_BL(32, 16)
+  SPARC_BL(16, 16)
+#undef SPARC_BL
   return hipErrorInvalidValue;
 }
 
+// Two-point lanes exist for FP32 only; asked of FP64 (or of the fused MHD
+// kernel) the launch fails.
 template <typename T>
-hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int tx, int ty, const StageArgs<T>* runs,
-                              const int* item_start, int nrun, int nblk, hipStream_t s) {
-  if (mhd && zp) return launch_batch_tile<T, true, true, true>(tx, ty, runs, item_start, nrun, nblk, s);
-  if (mhd) return launch_batch_tile<T, true, true, false>(tx, ty, runs, item_start, nrun, nblk, s);
-  if (bg3d) return launch_batch_tile<T, false, true, false>(tx, ty, runs, item_start, nrun, nblk, s);
-  return launch_batch_tile<T, false, false, false>(tx, ty, runs, item_start, nrun, nblk, s);
+static bool lanes_ok(bool mhd, bool zp, int vw) {
+  return vw == 1 || (vw == 2 && sizeof(T) == 4 && (zp || !mhd));
 }
 
-template hipError_t launch_stage_batch<double>(bool, bool, bool, int, int, const StageArgs<double>*,
+template <typename T>
+hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty,
+                              const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
+                              hipStream_t s) {
+  if (!lanes_ok<T>(mhd, zp, vw)) return hipErrorInvalidValue;
+  if constexpr (sizeof(T) == 4) {
+    if (vw == 2) {
+      if (mhd) return launch_batch_tile<T, true, true, true, 2>(tx, ty, runs, item_start, nrun, nblk, s);
+      if (bg3d) return launch_batch_tile<T, false, true, false, 2>(tx, ty, runs, item_start, nrun, nblk, s);
+      return launch_batch_tile<T, false, false, false, 2>(tx, ty, runs, item_start, nrun, nblk, s);
+    }
+  }
+  if (mhd && zp) return launch_batch_tile<T, true, true, true, 1>(tx, ty, runs, item_start, nrun, nblk, s);
+  if (mhd) return launch_batch_tile<T, true, true, false, 1>(tx, ty, runs, item_start, nrun, nblk, s);
+  if (bg3d) return launch_batch_tile<T, false, true, false, 1>(tx, ty, runs, item_start, nrun, nblk, s);
+  return launch_batch_tile<T, false, false, false, 1>(tx, ty, runs, item_start, nrun, nblk, s);
+}
+
+template hipError_t launch_stage_batch<double>(bool, bool, bool, int, int, int, const StageArgs<double>*,
                                                const int*, int, int, hipStream_t);
-template hipError_t launch_stage_batch<float>(bool, bool, bool, int, int, const StageArgs<float>*,
+template hipError_t launch_stage_batch<float>(bool, bool, bool, int, int, int, const StageArgs<float>*,
                                               const int*, int, int, hipStream_t);
 
 // Persistent grid for a batch launch: the blocks of this kernel instance that
@@ -878,10 +960,10 @@ static int resident_grid(K kernel, int threads, int ncu) {
   return (n + 7) / 8 * 8;
 }
 
-template <typename T, bool MHD, bool BG3D, bool ZP>
+template <typename T, bool MHD, bool BG3D, bool ZP, int VW>
 static int batch_blocks_tile(int tx, int ty, int ncu) {
 #define SPARC_OCC(X, Y) \
-  if (tx == X && ty == Y) return resident_grid(k_stage_batch<T, MHD, BG3D, X, Y, ZP>, X * Y, ncu);
+  if (tx == X && ty == Y) return resident_grid(k_stage_batch<T, MHD, BG3D, X, Y, ZP, VW>, X * Y, ncu);
   SPARC_OCC(64, 4)
   SPARC_OCC(64, 8)
   SPARC_OCC(32, 8)
@@ -892,29 +974,46 @@ static int batch_blocks_tile(int tx, int ty, int ncu) {
 }
 
 template <typename T>
-static int batch_blocks_t(bool mhd, bool bg3d, bool zp, int tx, int ty, int ncu) {
-  if (mhd && zp) return batch_blocks_tile<T, true, true, true>(tx, ty, ncu);
-  if (mhd) return batch_blocks_tile<T, true, true, false>(tx, ty, ncu);
-  if (bg3d) return batch_blocks_tile<T, false, true, false>(tx, ty, ncu);
-  return batch_blocks_tile<T, false, false, false>(tx, ty, ncu);
+static int batch_blocks_t(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, int ncu) {
+  if constexpr (sizeof(T) == 4) {
+    if (vw == 2) {
+      if (mhd) return batch_blocks_tile<T, true, true, true, 2>(tx, ty, ncu);
+      if (bg3d) return batch_blocks_tile<T, false, true, false, 2>(tx, ty, ncu);
+      return batch_blocks_tile<T, false, false, false, 2>(tx, ty, ncu);
+    }
+  }
+  if (mhd && zp) return batch_blocks_tile<T, true, true, true, 1>(tx, ty, ncu);
+  if (mhd) return batch_blocks_tile<T, true, true, false, 1>(tx, ty, ncu);
+  if (bg3d) return batch_blocks_tile<T, false, true, false, 1>(tx, ty, ncu);
+  return batch_blocks_tile<T, false, false, false, 1>(tx, ty, ncu);
 }
 
-int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int tx, int ty, int ncu) {
-  return f32 ? batch_blocks_t<float>(mhd, bg3d, zp, tx, ty, ncu)
-             : batch_blocks_t<double>(mhd, bg3d, zp, tx, ty, ncu);
+int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, int ncu) {
+  return f32 ? batch_blocks_t<float>(mhd, bg3d, zp, vw, tx, ty, ncu)
+             : batch_blocks_t<double>(mhd, bg3d, zp, vw, tx, ty, ncu);
 }
 
 template <typename T>
-hipError_t launch_stage(bool mhd, bool bg3d, bool zp, int tx, int ty, const StageArgs<T>& A,
+hipError_t launch_stage(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, const StageArgs<T>& A,
                         hipStream_t s) {
-  if (mhd && zp) return launch_tile<T, true, true, true>(tx, ty, A, s);
-  if (mhd) return launch_tile<T, true, true, false>(tx, ty, A, s);
-  if (bg3d) return launch_tile<T, false, true, false>(tx, ty, A, s);
-  return launch_tile<T, false, false, false>(tx, ty, A, s);
+  if (!lanes_ok<T>(mhd, zp, vw)) return hipErrorInvalidValue;
+  if constexpr (sizeof(T) == 4) {
+    if (vw == 2) {
+      if (mhd) return launch_tile<T, true, true, true, 2>(tx, ty, A, s);
+      if (bg3d) return launch_tile<T, false, true, false, 2>(tx, ty, A, s);
+      return launch_tile<T, false, false, false, 2>(tx, ty, A, s);
+    }
+  }
+  if (mhd && zp) return launch_tile<T, true, true, true, 1>(tx, ty, A, s);
+  if (mhd) return launch_tile<T, true, true, false, 1>(tx, ty, A, s);
+  if (bg3d) return launch_tile<T, false, true, false, 1>(tx, ty, A, s);
+  return launch_tile<T, false, false, false, 1>(tx, ty, A, s);
 }
 
-template hipError_t launch_stage<double>(bool, bool, bool, int, int, const StageArgs<double>&, hipStream_t);
-template hipError_t launch_stage<float>(bool, bool, bool, int, int, const StageArgs<float>&, hipStream_t);
+template hipError_t launch_stage<double>(bool, bool, bool, int, int, int, const StageArgs<double>&,
+                                         hipStream_t);
+template hipError_t launch_stage<float>(bool, bool, bool, int, int, int, const StageArgs<float>&,
+                                        hipStream_t);
 
 bool stage_tile_supported(int tx, int ty) {
   return (tx == 64 && (ty == 4 || ty == 8)) || (tx == 32 && (ty == 8 || ty == 16)) ||

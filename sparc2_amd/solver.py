@@ -15,6 +15,7 @@ State arrays are numpy (nv, nz, ny, nx) FP64 == the reference's Fortran
 from __future__ import annotations
 
 import ctypes as C
+import os
 from typing import Dict, Optional, Sequence
 
 import numpy as np
@@ -66,6 +67,7 @@ class Solver:
         for q in range(3):
             c.tile[q] = int(tile[q])
         self._cfg = c
+        self._lanes_env = os.environ.get("SPARC_LANE_POINTS", "")
         h = C.c_void_p()
         check(L.sparc_hip_create(C.byref(c), C.byref(h)), "sparc_hip_create")
         self.h = h
@@ -245,10 +247,11 @@ class Solver:
 
     def tile_key(self) -> tuple:
         """What the best tile depends on: grid, physics variant, precision,
-        boundary kind and CPML widths."""
+        boundary kind, CPML widths and the points per lane (SPARC_LANE_POINTS
+        read when the context was made)."""
         c = self.cfg
         return (c.nx, c.ny, c.nz, c.magnetic, c.precision, bool(c.periodic), bool(c.use_hpml),
-                tuple(c.npml), bool(self.st.bg3d))
+                tuple(c.npml), bool(self.st.bg3d), self._lanes_env)
 
     def autotune(self, reps: int = 3, cache: bool = True):
         """Pick the fastest tile shape for this grid by timing a middle RK
