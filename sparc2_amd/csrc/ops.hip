@@ -11,6 +11,8 @@
 // 451-562) evaluate the 11-point sums in the reference's index order; they
 // serve the background set-up (initialize.f90:993-1016,1219-1228) and the
 // drop-in operator entry points.
+#include <algorithm>
+
 #include "sparc_dev.h"
 
 namespace sparc {
@@ -80,9 +82,81 @@ __global__ __launch_bounds__(256) void k_filter(const T* __restrict__ in, T* __r
   out[(long)f * fs + idx] = (G == 5 || bare) ? filt : sm(pos) - s * filt;
 }
 
+// z filter: one column per thread marching up a chunk of planes with a
+// register queue of the raw values (planes pos-G-1 .. pos+G+1: the filter
+// reads smoothed values at pos-G .. pos+G, and a smoothed value reads its two
+// neighbours), so each value is loaded once; loads run ZPD planes ahead.
+// Same per-point arithmetic, in the same order, as k_filter along z.
+constexpr int ZPD = 4;
+
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_filter_z(const T* __restrict__ in, T* __restrict__ out,
+                                                  int nx, int ny, int nz, long fs, int ncb, int kc,
+                                                  T s, int bare) {
+  const int nch = (nz + kc - 1) / kc;
+  const int lg = blockIdx.x;
+  const int cb = lg % ncb, rest = lg / ncb;
+  const int chunk = rest % nch, f = rest / nch;
+  const long nxy = (long)nx * ny;
+  const long col = (long)cb * 256 + threadIdx.x;
+  if (col >= nxy) return;
+  const T* line = in + (long)f * fs + col;
+  T* o = out + (long)f * fs + col;
+  const int k0 = chunk * kc, k1 = min(k0 + kc, nz);
+  auto ld = [&](int r) -> T { return (r >= 0 && r < nz) ? line[(long)r * nxy] : T(0); };
+  constexpr int Q = 2 * G + 3;
+  T q[Q], pf[ZPD];
+#pragma unroll
+  for (int m = 0; m < Q - 1; ++m) q[m] = ld(k0 - G - 1 + m);
+#pragma unroll
+  for (int d = 0; d < ZPD; ++d) pf[d] = ld(k0 + G + 1 + d);
+  for (int pos = k0; pos < k1; ++pos) {
+    q[Q - 1] = pf[0];
+#pragma unroll
+    for (int d = 0; d < ZPD - 1; ++d) pf[d] = pf[d + 1];
+    pf[ZPD - 1] = ld(pos + G + 1 + ZPD);
+    // smoothed value of plane pos+d (derivatives.f90:690-693)
+    auto sm = [&](int d) -> T {
+      const int r = pos + d;
+      if (!bare && ((r >= 1 && r <= G - 1) || (r >= nz - G && r <= nz - 2)))
+        return (q[d + G] + q[d + G + 2]) / T(2);
+      return q[d + G + 1];
+    };
+    T filt;
+    if (pos < G || pos >= nz - G) {
+      filt = sm(0);  // z: copy for both widths (derivatives.f90:624,635)
+    } else {
+      T acc = 0;
+#pragma unroll
+      for (int m = -G; m <= G; ++m) acc += sm(m) * filt_w<T>(G, m);
+      filt = acc;
+    }
+    o[(long)pos * nxy] = (G == 5 || bare) ? filt : sm(0) - s * filt;
+#pragma unroll
+    for (int m = 0; m < Q - 1; ++m) q[m] = q[m + 1];
+  }
+}
+
 template <typename T>
 hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
                          int nfield, int periodic, T s, int bare, hipStream_t st) {
+  if (axis == 2) {
+    // column blocks x z chunks x fields: >= ~2048 blocks, chunks >= 16 deep
+    const long nxy = (long)nx * ny;
+    const int ncb = (int)((nxy + 255) / 256);
+    const long want = (2048 + (long)ncb * nfield - 1) / ((long)ncb * nfield);
+    const int nch = (int)std::max(1L, std::min(want, (long)nz / 16));
+    const int kc = (nz + nch - 1) / nch;
+    const long nb = (long)ncb * ((nz + kc - 1) / kc) * nfield;
+    if (nb >= (1L << 31)) return hipErrorInvalidValue;
+    if (g == 5)
+      hipLaunchKernelGGL((k_filter_z<T, 5>), dim3((unsigned)nb), dim3(256), 0, st, in, out, nx, ny,
+                         nz, fs, ncb, kc, s, bare);
+    else
+      hipLaunchKernelGGL((k_filter_z<T, 3>), dim3((unsigned)nb), dim3(256), 0, st, in, out, nx, ny,
+                         nz, fs, ncb, kc, s, bare);
+    return hipGetLastError();
+  }
   // one block per x-row segment: 256 threads, or the row rounded up to
   // whole waves when it is shorter
   const int nt = nx >= 256 ? 256 : (nx + 63) / 64 * 64;
