@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import sys
 import time
 from typing import Callable, Dict, List, Optional
 
@@ -119,14 +120,24 @@ def poisson_stream(n: int, rate: float = 4.0, seed: int = 7, precision: str = "f
 
 
 class Server:
+    """Continuous-batching server.  A request's set-up -- background, CFL
+    time step, FP64 set-up and its device context -- runs on a worker thread
+    (``setup_workers`` of them) while the admitted runs advance; its HBM
+    footprint is reserved when its set-up starts, in arrival order, and it
+    joins the batch at the first advance after its set-up has finished.  While
+    set-ups are in flight the advances are cut to ``poll_chunk`` steps so a
+    ready run joins promptly."""
+
     def __init__(self, hbm_budget: float = 200e9, max_chunk: int = 200, use_graph: bool = True,
-                 device: int = 0, keep_final: bool = False,
-                 clock: Callable[[], float] = time.perf_counter):
+                 device: int = 0, keep_final: bool = False, setup_workers: int = 2,
+                 poll_chunk: int = 25, clock: Callable[[], float] = time.perf_counter):
         self.budget = hbm_budget
         self.max_chunk = max_chunk
         self.use_graph = use_graph
         self.device = device
         self.keep_final = keep_final
+        self.setup_workers = max(1, setup_workers)
+        self.poll_chunk = max(1, poll_chunk)
         self.clock = clock
 
     def _start(self, req: Request):
@@ -139,10 +150,12 @@ class Server:
         return {"solver": s, "done": 0, "every": every, "k0": surface_plane(bg["zaxis"])}
 
     def run(self, requests: List[Request], verbose: bool = False) -> Dict:
+        from concurrent.futures import ThreadPoolExecutor
         from .batch import Batch
         reqs = sorted(requests, key=lambda r: r.arrival)
         results = {r.rid: Result(rid=r.rid, arrival=r.arrival) for r in reqs}
-        waiting: List[Request] = []
+        waiting: List[Request] = []    # arrived, set-up not started
+        pending: List[tuple] = []      # (request, footprint, future): set-up running
         active: Dict[int, dict] = {}
         used = 0.0
         t0 = self.clock()
@@ -150,65 +163,78 @@ class Server:
         nxt = 0
         batch, members = None, ()
         busy = 0.0
-        while nxt < len(reqs) or waiting or active:
-            while nxt < len(reqs) and reqs[nxt].arrival <= now():
-                waiting.append(reqs[nxt])
-                nxt += 1
-            # admission in arrival order while the footprint fits
-            while waiting:
-                fp = device_footprint(waiting[0].cfg)
-                if active and used + fp > self.budget:
-                    break
-                req = waiting.pop(0)
-                st = self._start(req)
-                st["req"], st["fp"] = req, fp
-                active[req.rid] = st
-                used += fp
-                results[req.rid].admitted = now()
-                results[req.rid].points = req.cfg.npoints
-                if verbose:
-                    print(f"[serve] t={now():7.2f}s admit {req.rid} {req.cfg.nx}x{req.cfg.ny}x{req.cfg.nz} "
-                          f"{'MHD' if req.cfg.mhd else 'quiet'} {req.nsteps} steps", flush=True)
-            if not active:
-                if nxt < len(reqs):
-                    time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
-                continue
-            key = tuple(sorted(active))
-            if key != members:
-                if batch is not None:
-                    batch.close()
-                sol = [active[r]["solver"] for r in key]
-                batch = Batch(sol, use_graph=self.use_graph)
-                batch.time = [active[r]["done"] for r in key]
-                members = key
-            # advance to the next event of any run: a slice or its end
-            chunk = self.max_chunk
-            for r in key:
-                a = active[r]
-                left = a["req"].nsteps - a["done"]
-                to_slice = a["every"] - (a["done"] % a["every"])
-                chunk = min(chunk, left, to_slice)
-            tb = self.clock()
-            batch.advance(chunk)
-            busy += self.clock() - tb
-            for r in key:
-                a = active[r]
-                a["done"] += chunk
-                res = results[r]
-                res.steps = a["done"]
-                if a["done"] % a["every"] == 0:
-                    res.slices.append((a["done"], a["solver"].get_slab(VZ, a["k0"], a["k0"])[0]))
-                if a["done"] >= a["req"].nsteps:
-                    if self.keep_final:
-                        res.final = a["solver"].get_state()
-                    res.finished = now()
-                    a["solver"].close()
-                    used -= a["fp"]
-                    del active[r]
-                    if verbose:
-                        print(f"[serve] t={now():7.2f}s done  {r} latency {res.latency:.2f}s", flush=True)
-        if batch is not None:
-            batch.close()
+        say = (lambda m: print(m, file=sys.stderr, flush=True)) if verbose else (lambda m: None)
+        pool = ThreadPoolExecutor(max_workers=self.setup_workers)
+        try:
+            while nxt < len(reqs) or waiting or pending or active:
+                while nxt < len(reqs) and reqs[nxt].arrival <= now():
+                    waiting.append(reqs[nxt])
+                    nxt += 1
+                # reserve HBM and start set-ups in arrival order while the footprint fits
+                while waiting:
+                    fp = device_footprint(waiting[0].cfg)
+                    if (active or pending) and used + fp > self.budget:
+                        break
+                    req = waiting.pop(0)
+                    used += fp
+                    pending.append((req, fp, pool.submit(self._start, req)))
+                # nothing on the device: wait for the oldest set-up
+                if not active and pending:
+                    pending[0][2].result()
+                still = []
+                for req, fp, fut in pending:
+                    if not fut.done():
+                        still.append((req, fp, fut))
+                        continue
+                    st = fut.result()
+                    st["req"], st["fp"] = req, fp
+                    active[req.rid] = st
+                    results[req.rid].admitted = now()
+                    results[req.rid].points = req.cfg.npoints
+                    say(f"[serve] t={now():7.2f}s admit {req.rid} {req.cfg.nx}x{req.cfg.ny}x{req.cfg.nz} "
+                        f"{'MHD' if req.cfg.mhd else 'quiet'} {req.nsteps} steps")
+                pending = still
+                if not active:
+                    if nxt < len(reqs):
+                        time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
+                    continue
+                key = tuple(sorted(active))
+                if key != members:
+                    if batch is not None:
+                        batch.close()
+                    sol = [active[r]["solver"] for r in key]
+                    batch = Batch(sol, use_graph=self.use_graph)
+                    batch.time = [active[r]["done"] for r in key]
+                    members = key
+                # advance to the next event of any run: a slice or its end
+                chunk = self.poll_chunk if (pending or waiting) else self.max_chunk
+                for r in key:
+                    a = active[r]
+                    left = a["req"].nsteps - a["done"]
+                    to_slice = a["every"] - (a["done"] % a["every"])
+                    chunk = min(chunk, left, to_slice)
+                tb = self.clock()
+                batch.advance(chunk)
+                busy += self.clock() - tb
+                for r in key:
+                    a = active[r]
+                    a["done"] += chunk
+                    res = results[r]
+                    res.steps = a["done"]
+                    if a["done"] % a["every"] == 0:
+                        res.slices.append((a["done"], a["solver"].get_slab(VZ, a["k0"], a["k0"])[0]))
+                    if a["done"] >= a["req"].nsteps:
+                        if self.keep_final:
+                            res.final = a["solver"].get_state()
+                        res.finished = now()
+                        a["solver"].close()
+                        used -= a["fp"]
+                        del active[r]
+                        say(f"[serve] t={now():7.2f}s done  {r} latency {res.latency:.2f}s")
+        finally:
+            pool.shutdown(wait=True)
+            if batch is not None:
+                batch.close()
         wall = now()
         lat = np.array([results[r.rid].latency for r in reqs])
         pts = sum(results[r.rid].points * results[r.rid].steps for r in reqs)
