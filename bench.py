@@ -79,10 +79,11 @@ def alg_bytes_step(cfg, w):
     return b
 
 
-def profiled_traffic(workload, precision, tile):
+def profiled_traffic(workload, precision, tile, lane_points=1):
     """HBM bytes per launch of the stage kernel, from the committed rocprofv3
     summary profiles/<workload>_<precision>.md (FETCH_SIZE and WRITE_SIZE
-    scaled by a 1 GiB copy), when it was taken with the same tile shape."""
+    scaled by a 1 GiB copy), when it was taken with the same tile shape and
+    points per lane."""
     path = os.path.join(ROOT, "profiles", f"{workload}_{precision}.md")
     if not os.path.exists(path):
         return None, None
@@ -92,7 +93,7 @@ def profiled_traffic(workload, precision, tile):
         summ = json.loads(block)
     except (IndexError, ValueError):
         return None, None
-    if list(summ.get("tile") or []) != list(tile[:2]):
+    if list(summ.get("tile") or []) != list(tile[:2]) or summ.get("lane_points", 1) != lane_points:
         return None, None
     k = summ["kernels"].get("k_stage")
     return (k["traffic_bytes_per_launch"] if k else None), os.path.relpath(path, ROOT)
@@ -359,7 +360,10 @@ def main():
     log(f"[bench] RK stage {stage_ms:.4f} ms, {bytes_stage / 1e9:.3f} GB minimum -> {stage_frac:.4f} of peak")
     step_bytes = alg_bytes_step(cfg, w)
     tile_used, nblk = s.tile()
-    traffic, traffic_src = profiled_traffic(args.workload, args.precision, tile_used)
+    # the library gives FP32 lanes two x points on grids with even nx
+    lanes = 2 if (cfg.precision == "fp32" and cfg.nx % 2 == 0
+                  and os.environ.get("SPARC_LANE_POINTS", "") != "1") else 1
+    traffic, traffic_src = profiled_traffic(args.workload, args.precision, tile_used, lanes)
     s.close()
 
     cpu = None
