@@ -209,7 +209,10 @@ def mode_batch(args):
         "data": "synthetic (seeded population: polytrope / flux-tube backgrounds)",
         "config": {"workload": "batch_population", "runs": len(pop), "mhd_runs": n_mhd,
                    "points": pts, "groups": groups, "launch_blocks": blocks, "graph": True,
-                   "autotuned_tiles": [list(t) for t in tiles]},
+                   "autotuned_tiles": [list(t) for t in tiles],
+                   # per variant group (MHD, 3-D backgrounds): tile in lanes, z-chunk multiple
+                   "tuned_groups": ([[int(k[0]), int(k[1]), *v] for k, v in sorted(b.tiles.items())]
+                                    if b.tiles else None)},
         "plain_fp64_gpts": round(plain, 5) if plain else None,
         "speedup_vs_plain_fp64": round(pts * K / el / 1e9 / plain, 3) if plain else None}),
           flush=True)

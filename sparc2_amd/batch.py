@@ -37,35 +37,53 @@ def _time_stage(h, reps: int) -> float:
     return ms.value
 
 
+# z-chunk depths the batch tuner tries, as multiples of each run's automatic
+# chunk: a persistent block pays a fixed cost per work item (its staging
+# plan, the argument switch, a barrier), which deeper chunks spread over more
+# planes.
+KC_MULT = (1, 2, 4)
+
+
+def _apply(members: Sequence[Solver], tile, auto, mult):
+    for s, k in zip(members, auto):
+        s.set_tile((tile[0], tile[1], 0 if mult == 1 else min(k * mult, s.cfg.nz)))
+
+
 def tune_tiles(solvers: Sequence[Solver], reps: int = 3) -> Dict[tuple, tuple]:
     """Tile autotuning for a batch.  Runs that share a kernel variant share
     one launch, so the shape is chosen per variant group, by timing a middle
     RK stage of that group's batched launch for every shape that fits all of
-    its grids (timing leaves the runs' states unchanged).  Returns
-    {variant: (tx, ty)} and leaves each run on its group's best shape."""
+    its grids, each at the runs' automatic z chunks and at KC_MULT times
+    them (timing leaves the runs' states unchanged).  Returns
+    {variant: (tx, ty, chunk multiple)} and leaves each run on its group's
+    best choice."""
     groups: Dict[tuple, List[Solver]] = {}
     for s in solvers:
         groups.setdefault((s.cfg.magnetic == 2, bool(s.st.bg3d)), []).append(s)
     chosen = {}
     for key, members in groups.items():
-        best, best_ms = None, float("inf")
+        best, best_ms, best_auto = None, float("inf"), None
         for t in TILES:
             try:
                 for s in members:
                     s.set_tile((t[0], t[1], 0))
             except SparcError:
                 continue
-            h = _create(members, use_graph=False)
-            try:
-                ms = _time_stage(h, reps)
-            finally:
-                lib().sparc_hip_batch_destroy(h)
-            if ms < best_ms:
-                best, best_ms = t, ms
+            auto = [s.tile()[0][2] for s in members]
+            for mult in KC_MULT:
+                if mult > 1 and all(k * mult > s.cfg.nz for s, k in zip(members, auto)):
+                    break
+                _apply(members, t, auto, mult)
+                h = _create(members, use_graph=False)
+                try:
+                    ms = _time_stage(h, reps)
+                finally:
+                    lib().sparc_hip_batch_destroy(h)
+                if ms < best_ms:
+                    best, best_ms, best_auto = (t[0], t[1], mult), ms, auto
         if best is None:
             raise SparcError(f"no tile shape fits every grid of variant {key}")
-        for s in members:
-            s.set_tile((best[0], best[1], 0))
+        _apply(members, best[:2], best_auto, best[2])
         chosen[key] = best
     return chosen
 

@@ -909,6 +909,7 @@ struct BatchGroup {
   bool mhd = false, bg3d = false, zp = false;
   int vw = 1, tx = 0, ty = 0, nblk = 0, znblk = 0;
   std::vector<int> members;
+  std::vector<int> zkc;        // z-pass chunk depth per member in this batch
   void* args[5] = {};         // StageArgs<T>[members] per stage (device)
   int* item_start = nullptr;   // prefix of stage work items (device)
   int* zitem_start = nullptr;  // prefix of z-pass work items (device)
@@ -955,6 +956,10 @@ static int batch_upload_args(sparc_batch* b) {
         sparc_ctx* c = b->runs[r];
         fill_common<T>(c, h[q]);
         set_stage<T>(c, h[q], s);
+        if (g.zp) {
+          h[q].zkc = g.zkc[q];
+          h[q].znchunk = (c->nz + g.zkc[q] - 1) / g.zkc[q];
+        }
         h[q].amp_tab = c->cfg.pulse_dir ? (const T*)b->amp[r] : nullptr;
         h[q].tstep = b->tstep;
       }
@@ -1082,10 +1087,20 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
     g.nblk = std::min(stage_batch_blocks(b->f32, g.mhd, g.bg3d, g.zp, g.vw, g.tx, g.ty, ncu),
                       (start.back() + 7) / 8 * 8);
     if (g.zp) {
+      // z-pass chunks sized for the whole group rather than per run: about
+      // twice the resident grid in items, each chunk >= 48 planes deep (the
+      // 10-plane queue warm-up is re-read per chunk)
+      g.znblk = zpass_batch_blocks(b->f32, ncu);
+      long ncols = 0;
+      for (int r : g.members) ncols += zpass_items(b->runs[r]->nx, b->runs[r]->ny, 1);
+      const long per_run = std::max<long>(1, (2L * g.znblk + ncols - 1) / ncols);
       std::vector<int> zs(g.members.size() + 1, 0);
+      g.zkc.assign(g.members.size(), 0);
       for (size_t q = 0; q < g.members.size(); ++q) {
         const sparc_ctx* c = b->runs[g.members[q]];
-        zs[q + 1] = zs[q] + (int)zpass_items(c->nx, c->ny, c->znchunk);
+        const int kc = (int)std::min<long>(c->nz, std::max<long>(48, (c->nz + per_run - 1) / per_run));
+        g.zkc[q] = kc;
+        zs[q + 1] = zs[q] + (int)zpass_items(c->nx, c->ny, (c->nz + kc - 1) / kc);
       }
       if ((rc = dalloc((void**)&g.zitem_start, sizeof(int) * zs.size()))) return bail(rc);
       if (hipMemcpy(g.zitem_start, zs.data(), sizeof(int) * zs.size(), hipMemcpyHostToDevice) != hipSuccess)

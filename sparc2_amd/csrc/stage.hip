@@ -503,12 +503,35 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           gpz = v[0], dvzdz = v[1];
         }
       }
-      // x faces point by point: the points of a lane can straddle the face
-      // edge, and an x face row (n points) need not start on a lane boundary
+      // x faces: with two points per lane and an even face width and offset,
+      // a lane pair lies wholly inside or outside the face and its face
+      // element is even, so the pair goes through at once; otherwise point by
+      // point (a pair can straddle the face edge, and an x face row of n
+      // points need not start on a lane boundary)
 #pragma unroll
       for (int fx = XBOT; fx <= XTOP; ++fx) {
         const auto& F = Ap->face[fx];
         if (!F.active) continue;
+        if constexpr (VW == 2) {
+          if (((F.n | F.off) & 1) == 0) {
+            const int ip = i - F.off;
+            if (ip >= 0 && ip < F.n) {
+              const uint32_t fp = (uint32_t)(ip + F.n * (j + ny * k));
+              if constexpr (MHD) {
+                V v[6] = {gpx, dvxdx, dxf3, dxf2, dxby, dxbz};
+                constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
+                pml_face(F, fp, v, fam, first, last, bnext, dt);
+                gpx = v[0], dvxdx = v[1], dxf3 = v[2], dxf2 = v[3], dxby = v[4], dxbz = v[5];
+              } else {
+                V v[2] = {gpx, dvxdx};
+                constexpr int fam[2] = {1, 0};
+                pml_face(F, fp, v, fam, first, last, bnext, dt);
+                gpx = v[0], dvxdx = v[1];
+              }
+            }
+            continue;
+          }
+        }
 #pragma unroll
         for (int c = 0; c < VW; ++c) {
           const int ip = i + c - F.off;
@@ -835,8 +858,9 @@ __global__ __launch_bounds__(ZP_NT, 2) void k_zpass_batch(const StageArgs<T>* __
   const int total = item_start[nrun];
   const int xcd = blockIdx.x & 7, m = blockIdx.x >> 3, nb8 = gridDim.x >> 3;
   const int lo = (int)((long)total * xcd / 8), hi = (int)((long)total * (xcd + 1) / 8);
+  // a block's items increase, so its run index only moves forward
+  int r = 0;
   for (int item = lo + m; item < hi; item += nb8) {
-    int r = 0;
     while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
     zpass_item<T>((CArgs<T>)(runs + r), item - item_start[r]);
   }
@@ -864,8 +888,9 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP, VW>())) void
   const int total = item_start[nrun];
   const int xcd = blockIdx.x & 7, m = blockIdx.x >> 3, nb8 = gridDim.x >> 3;
   const int lo = (int)((long)total * xcd / 8), hi = (int)((long)total * (xcd + 1) / 8);
+  // a block's items increase, so its run index only moves forward
+  int r = 0;
   for (int item = lo + m; item < hi; item += nb8) {
-    int r = 0;
     while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
     // no kernel writes the argument array while the batch runs
     stage_item<T, MHD, BG3D, TX, TY, ZP, VW>((CArgs<T>)(runs + r), item - item_start[r]);

@@ -4,9 +4,10 @@ one point per lane (SPARC_LANE_POINTS=1), on the same seeded input.
 A two-point lane computes each of its points with the operations, in the
 order, of a one-point lane, so the states after N steps agree bitwise; the
 diagnostics sum their block partials in another order (relative 1e-12).
-The cases cover the x-edge closures and x-face CPML points a lane pair can
-straddle (non-periodic quiet with x/y CPML, six-face MHD), periodic wrap and
-an odd nx, which keeps one point per lane.
+The cases cover the x-edge closures, x-face CPML points a lane pair can
+straddle (odd face widths: point by point) and even face widths (the pair at
+once), for non-periodic quiet with x/y CPML and six-face MHD, periodic wrap,
+and an odd nx, which keeps one point per lane.
 """
 import os
 
@@ -47,6 +48,11 @@ CASES = {
                                     npml=(5, 5, 5, 5, 8, 8), precision="fp32"),
     "mhd_six_faces": lambda: mhd_cfg(nx=48, ny=40, nz=40, npml=(7, 7, 6, 6, 7, 7),
                                      precision="fp32"),
+    # even x-face widths: lane pairs go through the x-face CPML together
+    "quiet_hpml_even_x": lambda: quiet_cfg(nx=56, ny=40, nz=40, periodic=False, use_hpml=True,
+                                           npml=(6, 6, 5, 5, 8, 8), precision="fp32"),
+    "mhd_even_x": lambda: mhd_cfg(nx=48, ny=40, nz=40, npml=(10, 10, 7, 7, 7, 7),
+                                  precision="fp32"),
     "mhd_periodic": lambda: mhd_cfg(nx=64, ny=32, nz=40, periodic=True, precision="fp32"),
 }
 
