@@ -25,7 +25,7 @@ template <typename T>
 hipError_t launch_zpass(const StageArgs<T>& A, hipStream_t s);
 template <typename T>
 hipError_t launch_zpass_batch(const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
-                              hipStream_t s);
+                              int* work, hipStream_t s);
 long zpass_items(int nx, int ny, int nchunk);
 int zpass_batch_blocks(bool f32, int ncu);
 bool stage_tile_supported(int tx, int ty);
@@ -39,7 +39,7 @@ hipError_t launch_zero(void* p, long n, bool f32, hipStream_t st);
 template <typename T>
 hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty,
                               const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
-                              hipStream_t s);
+                              int* work, hipStream_t s);
 int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, int ncu);
 hipError_t launch_diag_reduce_batch(const void* jobs, int njobs, hipStream_t st);
 hipError_t launch_tick(int* c, int reset, hipStream_t st);
@@ -923,6 +923,9 @@ struct sparc_batch {
   std::vector<sparc_ctx*> runs;
   std::vector<BatchGroup> groups;
   int* tstep = nullptr;
+  // work counters of the persistent launches (8 per-XCD item counters and a
+  // count of the blocks that left; each launch leaves them zero)
+  int* work = nullptr;
   std::vector<void*> amp;  // per run, amp_cap entries (device)
   int amp_cap = 0;
   void* jobs = nullptr;    // DiagJob[nrun] (device)
@@ -942,6 +945,7 @@ static void batch_free(sparc_batch* b) {
   for (auto& p : b->amp)
     if (p) (void)hipFree(p);
   if (b->tstep) (void)hipFree(b->tstep);
+  if (b->work) (void)hipFree(b->work);
   if (b->jobs) (void)hipFree(b->jobs);
   if (b->st) (void)hipStreamDestroy(b->st);
 }
@@ -976,10 +980,10 @@ static int batch_enqueue_step(sparc_batch* b) {
       const StageArgs<T>* args = (const StageArgs<T>*)g.args[s - 1];
       const int nm = (int)g.members.size();
       hipError_t e = hipSuccess;
-      if (g.zp) e = launch_zpass_batch<T>(args, g.zitem_start, nm, g.znblk, b->st);
+      if (g.zp) e = launch_zpass_batch<T>(args, g.zitem_start, nm, g.znblk, b->work, b->st);
       if (e == hipSuccess)
         e = launch_stage_batch<T>(g.mhd, g.bg3d, g.zp, g.vw, g.tx, g.ty, args, g.item_start, nm, g.nblk,
-                                  b->st);
+                                  b->work, b->st);
       if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch stage launch: ") + hipGetErrorString(e));
     }
   if (launch_diag_reduce_batch(b->jobs, (int)b->runs.size(), b->st) != hipSuccess ||
@@ -1109,6 +1113,9 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
     }
   }
   if ((rc = dalloc((void**)&b->tstep, sizeof(int)))) return bail(rc);
+  if ((rc = dalloc((void**)&b->work, 16 * sizeof(int)))) return bail(rc);
+  if (launch_zero(b->work, 16, true, b->st) != hipSuccess || hipStreamSynchronize(b->st) != hipSuccess)
+    return bail(fail(SPARC_E_HIP, "work counter clear failed"));
   std::vector<DiagJob> jobs(nrun);
   for (int r = 0; r < nrun; ++r) {
     sparc_ctx* c = b->runs[r];
@@ -1208,10 +1215,10 @@ static int batch_time_stage_t(sparc_batch* b, int reps, double* ms) {
       if (e != hipSuccess) break;
       const StageArgs<T>* args = (const StageArgs<T>*)g.args[1];
       const int nm = (int)g.members.size();
-      if (g.zp) e = launch_zpass_batch<T>(args, g.zitem_start, nm, g.znblk, b->st);
+      if (g.zp) e = launch_zpass_batch<T>(args, g.zitem_start, nm, g.znblk, b->work, b->st);
       if (e == hipSuccess)
         e = launch_stage_batch<T>(g.mhd, g.bg3d, g.zp, g.vw, g.tx, g.ty, args, g.item_start, nm, g.nblk,
-                                  b->st);
+                                  b->work, b->st);
     }
   }
   if (e == hipSuccess) e = hipEventRecord(e1, b->st);

@@ -852,18 +852,47 @@ __global__ __launch_bounds__(ZP_NT, 2) void k_zpass(const StageArgs<T> A) {
   zpass_item<T>((CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr(), logical);
 }
 
+// Work distribution of the persistent batch kernels: the item list is cut
+// into eight contiguous ranges, one per XCD (blocks with blockIdx % 8 == x,
+// which the hardware places on XCD x), and the blocks of a range take its
+// items in order from an atomic counter -- work[x] -- so a block that drew
+// cheap items (no CPML face, no edge closure) takes more of them.  The last
+// block to leave zeroes the counters for the next launch (work[8] counts the
+// blocks that left), so no memset is needed between launches or graph
+// replays.  Returns the next item of this block's range, or -1.
+__device__ __forceinline__ int next_item(int* work, int lo, int hi, int* slot) {
+  __syncthreads();  // every thread has read the previous item
+  if (threadIdx.x == 0) *slot = lo + atomicAdd(&work[blockIdx.x & 7], 1);
+  __syncthreads();
+  const int item = *slot;
+  return item < hi ? item : -1;
+}
+
+__device__ __forceinline__ void leave(int* work) {
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&work[8], 1) == (int)gridDim.x - 1) {
+      for (int x = 0; x < 9; ++x) atomicExch(&work[x], 0);
+      __threadfence();
+    }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(ZP_NT, 2) void k_zpass_batch(const StageArgs<T>* __restrict__ runs,
-                                                           const int* __restrict__ item_start, int nrun) {
+                                                           const int* __restrict__ item_start, int nrun,
+                                                           int* __restrict__ work) {
+  __shared__ int slot;
   const int total = item_start[nrun];
-  const int xcd = blockIdx.x & 7, m = blockIdx.x >> 3, nb8 = gridDim.x >> 3;
+  const int xcd = blockIdx.x & 7;
   const int lo = (int)((long)total * xcd / 8), hi = (int)((long)total * (xcd + 1) / 8);
   // a block's items increase, so its run index only moves forward
   int r = 0;
-  for (int item = lo + m; item < hi; item += nb8) {
+  for (int item = next_item(work, lo, hi, &slot); item >= 0; item = next_item(work, lo, hi, &slot)) {
     while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
     zpass_item<T>((CArgs<T>)(runs + r), item - item_start[r]);
   }
+  leave(work);
 }
 
 // Single run: one block per work item.  The hardware hands consecutive block
@@ -879,23 +908,25 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP, VW>())) void
 }
 
 // Many runs in one persistent launch: the work items of all runs form one
-// list (item_start[r] = first item of run r, item_start[nrun] = total).  Each
-// XCD takes a contiguous eighth of the list and its blocks stride through it.
-// The grid size must be a multiple of 8.
+// list (item_start[r] = first item of run r, item_start[nrun] = total),
+// handed out per XCD range by next_item.  The grid size must be a multiple
+// of 8.
 template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP, int VW>
 __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP, VW>())) void k_stage_batch(
-    const StageArgs<T>* __restrict__ runs, const int* __restrict__ item_start, int nrun) {
+    const StageArgs<T>* __restrict__ runs, const int* __restrict__ item_start, int nrun,
+    int* __restrict__ work) {
+  __shared__ int slot;
   const int total = item_start[nrun];
-  const int xcd = blockIdx.x & 7, m = blockIdx.x >> 3, nb8 = gridDim.x >> 3;
+  const int xcd = blockIdx.x & 7;
   const int lo = (int)((long)total * xcd / 8), hi = (int)((long)total * (xcd + 1) / 8);
   // a block's items increase, so its run index only moves forward
   int r = 0;
-  for (int item = lo + m; item < hi; item += nb8) {
+  for (int item = next_item(work, lo, hi, &slot); item >= 0; item = next_item(work, lo, hi, &slot)) {
     while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
     // no kernel writes the argument array while the batch runs
     stage_item<T, MHD, BG3D, TX, TY, ZP, VW>((CArgs<T>)(runs + r), item - item_start[r]);
-    __syncthreads();
   }
+  leave(work);
 }
 
 // ---------------------------------------------------------------------------
@@ -922,17 +953,18 @@ static hipError_t launch_tile(int tx, int ty, const StageArgs<T>& A, hipStream_t
 
 template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP, int VW>
 static hipError_t launch_batch_one(const StageArgs<T>* runs, const int* item_start, int nrun,
-                                   int nblk, hipStream_t s) {
+                                   int nblk, int* work, hipStream_t s) {
   hipLaunchKernelGGL((k_stage_batch<T, MHD, BG3D, TX, TY, ZP, VW>), dim3(nblk), dim3(TX * TY), 0, s,
-                     runs, item_start, nrun);
+                     runs, item_start, nrun, work);
   return hipGetLastError();
 }
 
 template <typename T, bool MHD, bool BG3D, bool ZP, int VW>
 static hipError_t launch_batch_tile(int tx, int ty, const StageArgs<T>* runs, const int* item_start,
-                                    int nrun, int nblk, hipStream_t s) {
-#define SPARC_BL(X, Y) \
-  if (tx == X && ty == Y) return launch_batch_one<T, MHD, BG3D, X, Y, ZP, VW>(runs, item_start, nrun, nblk, s);
+                                    int nrun, int nblk, int* work, hipStream_t s) {
+#define SPARC_BL(X, Y)                                                                               \
+  if (tx == X && ty == Y)                                                                            \
+    return launch_batch_one<T, MHD, BG3D, X, Y, ZP, VW>(runs, item_start, nrun, nblk, work, s);
   SPARC_BL(64, 4)
   SPARC_BL(64, 8)
   SPARC_BL(32, 8)
@@ -952,25 +984,27 @@ static bool lanes_ok(bool mhd, bool zp, int vw) {
 template <typename T>
 hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty,
                               const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
-                              hipStream_t s) {
-  if (!lanes_ok<T>(mhd, zp, vw)) return hipErrorInvalidValue;
+                              int* work, hipStream_t s) {
+  if (!lanes_ok<T>(mhd, zp, vw) || nblk % 8 != 0) return hipErrorInvalidValue;
+#define SPARC_BT(M, B, Z, V) launch_batch_tile<T, M, B, Z, V>(tx, ty, runs, item_start, nrun, nblk, work, s)
   if constexpr (sizeof(T) == 4) {
     if (vw == 2) {
-      if (mhd) return launch_batch_tile<T, true, true, true, 2>(tx, ty, runs, item_start, nrun, nblk, s);
-      if (bg3d) return launch_batch_tile<T, false, true, false, 2>(tx, ty, runs, item_start, nrun, nblk, s);
-      return launch_batch_tile<T, false, false, false, 2>(tx, ty, runs, item_start, nrun, nblk, s);
+      if (mhd) return SPARC_BT(true, true, true, 2);
+      if (bg3d) return SPARC_BT(false, true, false, 2);
+      return SPARC_BT(false, false, false, 2);
     }
   }
-  if (mhd && zp) return launch_batch_tile<T, true, true, true, 1>(tx, ty, runs, item_start, nrun, nblk, s);
-  if (mhd) return launch_batch_tile<T, true, true, false, 1>(tx, ty, runs, item_start, nrun, nblk, s);
-  if (bg3d) return launch_batch_tile<T, false, true, false, 1>(tx, ty, runs, item_start, nrun, nblk, s);
-  return launch_batch_tile<T, false, false, false, 1>(tx, ty, runs, item_start, nrun, nblk, s);
+  if (mhd && zp) return SPARC_BT(true, true, true, 1);
+  if (mhd) return SPARC_BT(true, true, false, 1);
+  if (bg3d) return SPARC_BT(false, true, false, 1);
+  return SPARC_BT(false, false, false, 1);
+#undef SPARC_BT
 }
 
 template hipError_t launch_stage_batch<double>(bool, bool, bool, int, int, int, const StageArgs<double>*,
-                                               const int*, int, int, hipStream_t);
+                                               const int*, int, int, int*, hipStream_t);
 template hipError_t launch_stage_batch<float>(bool, bool, bool, int, int, int, const StageArgs<float>*,
-                                              const int*, int, int, hipStream_t);
+                                              const int*, int, int, int*, hipStream_t);
 
 // Persistent grid for a batch launch: the blocks of this kernel instance that
 // are resident at once (the runtime's occupancy for its registers and LDS),
@@ -1061,8 +1095,9 @@ hipError_t launch_zpass(const StageArgs<T>& A, hipStream_t s) {
 
 template <typename T>
 hipError_t launch_zpass_batch(const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
-                              hipStream_t s) {
-  hipLaunchKernelGGL((k_zpass_batch<T>), dim3(nblk), dim3(ZP_NT), 0, s, runs, item_start, nrun);
+                              int* work, hipStream_t s) {
+  if (nblk % 8 != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_zpass_batch<T>), dim3(nblk), dim3(ZP_NT), 0, s, runs, item_start, nrun, work);
   return hipGetLastError();
 }
 
@@ -1073,7 +1108,9 @@ int zpass_batch_blocks(bool f32, int ncu) {
 
 template hipError_t launch_zpass<double>(const StageArgs<double>&, hipStream_t);
 template hipError_t launch_zpass<float>(const StageArgs<float>&, hipStream_t);
-template hipError_t launch_zpass_batch<double>(const StageArgs<double>*, const int*, int, int, hipStream_t);
-template hipError_t launch_zpass_batch<float>(const StageArgs<float>*, const int*, int, int, hipStream_t);
+template hipError_t launch_zpass_batch<double>(const StageArgs<double>*, const int*, int, int, int*,
+                                               hipStream_t);
+template hipError_t launch_zpass_batch<float>(const StageArgs<float>*, const int*, int, int, int*,
+                                              hipStream_t);
 
 }  // namespace sparc
