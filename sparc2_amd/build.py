@@ -1,10 +1,12 @@
-"""Build libsparc_hip.so (the HIP kernels + C ABI) in-tree for gfx950.
+"""Build libsparc_hip.so (the HIP kernels + C ABI) in-tree for gfx950, and
+libsparc_h5.so (the reference's HDF5 file layouts, host code over the HDF5 C
+library) when HDF5 is installed.
 
     python -m sparc2_amd.build [--jobs N]
 
 Each .hip source is compiled to an object with hipcc, then linked into
 sparc2_amd/libsparc_hip.so.  Packed FP32 instructions are disabled for all
-device code.
+device code.  HDF5 is looked up under $SPARC_HDF5_DIR (default /opt/conda).
 """
 from __future__ import annotations
 
@@ -18,6 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libsparc_hip.so")
+H5LIB = os.path.join(HERE, "libsparc_h5.so")
+HDF5_DIR = os.environ.get("SPARC_HDF5_DIR", "/opt/conda")
 SOURCES = ["api.hip", "stage.hip", "ops.hip"]
 ARCH = os.environ.get("SPARC_OFFLOAD_ARCH", "gfx950")
 
@@ -42,6 +46,28 @@ def _compile(src: str) -> str:
     return obj
 
 
+def build_h5(verbose: bool = True):
+    """libsparc_h5.so from csrc/h5io.cpp with g++ (no device code); returns
+    its path, or None when the HDF5 headers and libraries are absent."""
+    inc, libdir = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and
+            os.path.exists(os.path.join(libdir, "libhdf5_hl.so"))):
+        if verbose:
+            print(f"HDF5 not found under {HDF5_DIR}: libsparc_h5.so not built")
+        return None
+    tmp = H5LIB + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", f"-I{inc}",
+           os.path.join(CSRC, "h5io.cpp"), f"-L{libdir}", "-lhdf5_hl", "-lhdf5",
+           f"-Wl,-rpath,{libdir}", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for h5io.cpp:\n{r.stderr}")
+    os.replace(tmp, H5LIB)
+    if verbose:
+        print(f"built {H5LIB}")
+    return H5LIB
+
+
 def build(jobs: int = 4, verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
@@ -54,6 +80,7 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_h5(verbose)
     return LIB
 
 

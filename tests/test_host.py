@@ -116,6 +116,18 @@ def test_library_exports_every_declared_entry_point():
     assert set(declared) == set(_lib.EXPORTS)
 
 
+def test_h5_library_exports_every_declared_entry_point():
+    from sparc2_amd import h5io
+    if not os.path.exists(h5io.LIB):
+        pytest.skip("libsparc_h5.so not built (no HDF5)")
+    hdr = open(os.path.join(ROOT, "include", "sparc_h5.h")).read()
+    declared = sorted(set(re.findall(r"^(?:int|const char\*)\s+(sparc_h5_\w+)\(", hdr, re.M)))
+    assert len(declared) == 6
+    L = ctypes.CDLL(h5io.LIB)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
 def test_library_refuses_to_run_without_a_device():
     """No silent CPU fallback: creating a context without a HIP device fails."""
     if not os.path.exists(_lib.LIB_PATH):
