@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
 import sys
 import time
 from typing import Callable, Dict, List, Optional
@@ -126,11 +127,15 @@ class Server:
     footprint is reserved when its set-up starts, in arrival order, and it
     joins the batch at the first advance after its set-up has finished.  While
     set-ups are in flight the advances are cut to ``poll_chunk`` steps so a
-    ready run joins promptly."""
+    ready run joins promptly.  With ``chunk_dir`` each returned slice is also
+    written there as a chunk file of the surface plane (all fields, the
+    layout of WRITE_OUT_CHUNK, all_modules.f90:1566-1799), named
+    ``req<rid>_chunk_NNNN.h5`` with NNNN the slice count."""
 
     def __init__(self, hbm_budget: float = 200e9, max_chunk: int = 200, use_graph: bool = True,
                  device: int = 0, keep_final: bool = False, setup_workers: int = 2,
-                 poll_chunk: int = 25, clock: Callable[[], float] = time.perf_counter):
+                 poll_chunk: int = 25, chunk_dir: Optional[str] = None,
+                 clock: Callable[[], float] = time.perf_counter):
         self.budget = hbm_budget
         self.max_chunk = max_chunk
         self.use_graph = use_graph
@@ -138,6 +143,7 @@ class Server:
         self.keep_final = keep_final
         self.setup_workers = max(1, setup_workers)
         self.poll_chunk = max(1, poll_chunk)
+        self.chunk_dir = chunk_dir
         self.clock = clock
 
     def _start(self, req: Request):
@@ -223,6 +229,11 @@ class Server:
                     res.steps = a["done"]
                     if a["done"] % a["every"] == 0:
                         res.slices.append((a["done"], a["solver"].get_slab(VZ, a["k0"], a["k0"])[0]))
+                        if self.chunk_dir:
+                            from . import h5io
+                            name = h5io.file_name(f"req{r}_", "chunk", len(res.slices))
+                            h5io.write_chunk(os.path.join(self.chunk_dir, name), a["solver"], a["done"],
+                                             zr=(a["k0"], a["k0"]))
                     if a["done"] >= a["req"].nsteps:
                         if self.keep_final:
                             res.final = a["solver"].get_state()

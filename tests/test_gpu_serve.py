@@ -2,6 +2,7 @@
 result (photospheric vz slices and final state) is the same as the request
 served alone (bitwise: the batched launch runs the same per-tile code)."""
 import copy
+import os
 
 import numpy as np
 import pytest
@@ -23,10 +24,12 @@ def _requests():
             for q, (t, c, z, n, b) in enumerate(specs)]
 
 
-def test_served_results_equal_alone():
-    from sparc2_amd.serve import Server
+def test_served_results_equal_alone(tmp_path):
+    from sparc2_amd import h5io
+    from sparc2_amd.serve import Server, cfl_timestep
     reqs = _requests()
-    mixed = Server(max_chunk=25, keep_final=True).run(copy.deepcopy(reqs), verbose=True)
+    chunks = str(tmp_path) if os.path.exists(h5io.LIB) else None
+    mixed = Server(max_chunk=25, keep_final=True, chunk_dir=chunks).run(copy.deepcopy(reqs), verbose=True)
     for r in reqs:
         alone = Server(max_chunk=25, keep_final=True).run([copy.deepcopy(r)])
         a = mixed["results"][r.rid]
@@ -37,5 +40,19 @@ def test_served_results_equal_alone():
             assert ta == tb
             np.testing.assert_array_equal(sa, sb)
         np.testing.assert_array_equal(a.final, b.final)
+        if chunks:
+            # each slice also went out as a chunk file of the surface plane:
+            # vz in physical units (x dimc, the sound speed at the first
+            # point), time in hours of the run's CFL time step
+            bg = r.build_background()
+            dimc = float(bg["cs0"][0, 0, 0])
+            dt = cfl_timestep(r.cfg, bg)
+            for n, (t, sl) in enumerate(a.slices, start=1):
+                path = os.path.join(chunks, h5io.file_name(f"req{r.rid}_", "chunk", n))
+                dims, hours = h5io.attrs(path)
+                assert dims == (r.cfg.nx, r.cfg.ny, 1)
+                assert hours == pytest.approx(t * dt / 3600.0, rel=1e-14)
+                vz = h5io.read(path, "vz", (1, r.cfg.ny, r.cfg.nx))[0]
+                np.testing.assert_allclose(vz, sl * dimc, rtol=1e-15, atol=0)
     print({k: v for k, v in mixed.items() if k != "results"}, flush=True)
     assert mixed["latency_p99"] >= mixed["latency_p50"] > 0
