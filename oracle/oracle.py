@@ -81,6 +81,9 @@ def lib():
         L.orc_diag.argtypes = [vp, P]
         L.orc_run.argtypes = [vp, C.c_int, C.c_int, P]
         L.orc_rhs_stage.argtypes = [vp, C.c_int]
+        L.orc_set_damping.argtypes = [vp, C.c_int]
+        L.orc_set_damping.restype = C.c_int
+        L.orc_damp_term.argtypes = [vp, P, P]
         for n in ("orc_ddx", "orc_ddy", "orc_ddz"):
             getattr(L, n).argtypes = [vp, P, P, C.c_int]
             getattr(L, n).restype = C.c_int
@@ -152,12 +155,26 @@ class Oracle:
         self.nv = L.orc_nv(self.h)
         self.npsi = L.orc_npsi(self.h)
         self.shape = (cfg.nz, cfg.ny, cfg.nx)
+        if getattr(cfg, "damping", False):
+            self.set_damping(True)
 
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
             lib().orc_destroy(h)
             self.h = None
+
+    # -- DAMPING --
+    def set_damping(self, on: bool):
+        if lib().orc_set_damping(self.h, int(on)) != 0:
+            raise MemoryError("damping rates")
+
+    def damp_term(self, v):
+        """damp_velocity of one (nz, ny, nx) field with the current rates."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros_like(v)
+        lib().orc_damp_term(self.h, _p(v), _p(out))
+        return out
 
     # -- set-up state --
     def scalar(self, name):

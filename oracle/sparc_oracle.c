@@ -148,6 +148,7 @@ struct orc_ctx {
   double *pvars[6], *ppsi[6], *pscr[6];
   double *a, *u, *scr, *forcing;
   double *w[NWORK];
+  double* damp_rates; /* (ny, nx/2+1) when DAMPING is on, else NULL */
 };
 
 #define IX(c, i, j, k) ((long)(i) + (long)(c)->nx * ((long)(j) + (long)(c)->ny * (long)(k)))
@@ -778,6 +779,7 @@ orc_ctx* orc_create(const orc_cfg* cfg, const double* rho0_in, const double* p0_
 }
 
 void orc_destroy(orc_ctx* c) {
+  if (c) free(c->damp_rates);
   if (!c) return;
   double* ptrs[] = {c->xaxis, c->yaxis, c->zaxis, c->g, c->cq, c->stretch, c->unstretch,
                     c->spongez, c->rho0, c->p0, c->cs, c->c2, c->rhoinv, c->c2rho0,
@@ -838,6 +840,7 @@ static const double* named(const orc_ctx* c, const char* n, long* len) {
       {"gradrho0_y", c->gradrho0_y, N},   {"gradp0_x", c->gradp0_x, N},
       {"gradp0_y", c->gradp0_y, N},       {"reduction", c->reduction, N},
       {"forcing", c->forcing, N},
+      {"damping_rates", c->damp_rates, (long)c->ny * (c->nx / 2 + 1)},
   };
   for (size_t q = 0; q < sizeof t / sizeof t[0]; ++q)
     if (!strcmp(n, t[q].name)) {
@@ -934,6 +937,133 @@ static void pml_apply(orc_ctx* c, int f, int q, double* d) {
 }
 
 /* RHS zeroing: RHS.f90:154-164, 647-657. */
+/* ------------------------------------------------------------------------ */
+/* FFT damping (DAMPING = 1).  INIT_DAMPING (initialize.f90:1276-1336) sets  */
+/* damping_rates(nx/2+1, ny); damp_velocity (all_modules.f90:1802-1888)      */
+/* returns, per z plane, the unnormalised c2r transform of the r2c transform */
+/* of the field times the rates; MP_QUIET_3D / MP_MHD_3D subtract it from    */
+/* the velocity RHS after the boundary zeroing (RHS.f90:166-172, 659-665).   */
+/* Here the transforms are direct DFT sums (separable, FP64): the r2c half   */
+/* spectrum extended by Hermitian symmetry, the c2r the real part of the     */
+/* full inverse sum.                                                         */
+/* ------------------------------------------------------------------------ */
+int orc_set_damping(orc_ctx* c, int on) {
+  free(c->damp_rates);
+  c->damp_rates = NULL;
+  if (!on) return 0;
+  const int nx = c->nx, ny = c->ny, mx = nx / 2 + 1;
+  c->damp_rates = dalloc((long)mx * ny);
+  if (!c->damp_rates) return -1;
+  const double constx = 2.0 * PI * RSUN / c->cfg.xlength;
+  const double consty = 2.0 * PI * RSUN / c->cfg.ylength;
+  const double kayref = 902.0;
+  for (int jj = 1; jj <= ny; ++jj) {
+    /* rows jj <= ny/2+1 use jj-1, the rest ny-jj+1 (initialize.f90:1300-1310) */
+    const double kyv = jj <= ny / 2 + 1 ? (double)(jj - 1) : (double)(ny - jj + 1);
+    for (int ii = 1; ii <= mx; ++ii) {
+      const double kxv = (double)(ii - 1);
+      const double kay = pow(constx * constx * kxv * kxv + consty * consty * kyv * kyv, 0.5);
+      c->damp_rates[(long)(jj - 1) * mx + (ii - 1)] =
+          c->diml / c->dimc / nx / ny * 1.0e-6 * (500.0 * pow(kay / kayref, 2.2)) * 20.0;
+    }
+  }
+  return 0;
+}
+
+void orc_damp_term(const orc_ctx* c, const double* v, double* term) {
+  const int nx = c->nx, ny = c->ny, mx = nx / 2 + 1;
+  const long nxy = (long)nx * ny;
+  /* twiddles of the transforms (the exact 2*pi, not the reference's pi) */
+  const double two_pi = 6.283185307179586476925286766559;
+  double *cx = dalloc(nx), *sx = dalloc(nx), *cy = dalloc(ny), *sy = dalloc(ny);
+  for (int q = 0; q < nx; ++q) {
+    cx[q] = cos(two_pi * q / nx);
+    sx[q] = sin(two_pi * q / nx);
+  }
+  for (int q = 0; q < ny; ++q) {
+    cy[q] = cos(two_pi * q / ny);
+    sy[q] = sin(two_pi * q / ny);
+  }
+  const double* R = c->damp_rates;
+#pragma omp parallel for
+  for (int k = 0; k < c->nz; ++k) {
+    const double* p = v + (long)k * nxy;
+    double* o = term + (long)k * nxy;
+    double *ar = dalloc(nxy), *ai = dalloc(nxy), *br = dalloc(nxy), *bi = dalloc(nxy);
+    /* forward along x: A(y, kx) */
+    for (int y = 0; y < ny; ++y)
+      for (int kx = 0; kx < nx; ++kx) {
+        double sr = 0.0, si = 0.0;
+        for (int x = 0; x < nx; ++x) {
+          const int q = (int)(((long)kx * x) % nx);
+          sr += p[(long)y * nx + x] * cx[q];
+          si -= p[(long)y * nx + x] * sx[q];
+        }
+        ar[(long)y * nx + kx] = sr;
+        ai[(long)y * nx + kx] = si;
+      }
+    /* forward along y and the rates: B(ky, kx) = rate * sum_y A e^{-i..} */
+    for (int ky = 0; ky < ny; ++ky)
+      for (int kx = 0; kx < nx; ++kx) {
+        double sr = 0.0, si = 0.0;
+        for (int y = 0; y < ny; ++y) {
+          const int q = (int)(((long)ky * y) % ny);
+          const double xr = ar[(long)y * nx + kx], xi = ai[(long)y * nx + kx];
+          sr += xr * cy[q] + xi * sy[q];
+          si += xi * cy[q] - xr * sy[q];
+        }
+        /* half spectrum kx <= nx/2; above it the Hermitian partner (nx-kx, ny-ky) */
+        const double r = kx < mx ? R[(long)ky * mx + kx] : R[(long)((ny - ky) % ny) * mx + (nx - kx)];
+        br[(long)ky * nx + kx] = sr * r;
+        bi[(long)ky * nx + kx] = si * r;
+      }
+    /* inverse along y: A(y, kx) = sum_ky B e^{+i..} */
+    for (int y = 0; y < ny; ++y)
+      for (int kx = 0; kx < nx; ++kx) {
+        double sr = 0.0, si = 0.0;
+        for (int ky = 0; ky < ny; ++ky) {
+          const int q = (int)(((long)ky * y) % ny);
+          const double xr = br[(long)ky * nx + kx], xi = bi[(long)ky * nx + kx];
+          sr += xr * cy[q] - xi * sy[q];
+          si += xi * cy[q] + xr * sy[q];
+        }
+        ar[(long)y * nx + kx] = sr;
+        ai[(long)y * nx + kx] = si;
+      }
+    /* inverse along x, real part */
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        double sr = 0.0;
+        for (int kx = 0; kx < nx; ++kx) {
+          const int q = (int)(((long)kx * x) % nx);
+          sr += ar[(long)y * nx + kx] * cx[q] - ai[(long)y * nx + kx] * sx[q];
+        }
+        o[(long)y * nx + x] = sr;
+      }
+    free(ar);
+    free(ai);
+    free(br);
+    free(bi);
+  }
+  free(cx);
+  free(sx);
+  free(cy);
+  free(sy);
+}
+
+/* RHSv -= damp_velocity(v) for the three velocities of the stage value. */
+static void damp_rhs(orc_ctx* c) {
+  if (!c->damp_rates) return;
+  const long N = c->N;
+  double* t = dalloc(N);
+  for (int f = 1; f <= 3; ++f) {
+    orc_damp_term(c, c->u + f * N, t);
+    double* s = c->scr + f * N;
+    for (long i = 0; i < N; ++i) s[i] = s[i] - t[i];
+  }
+  free(t);
+}
+
 static void zero_planes(orc_ctx* c) {
   const int nx = c->nx, ny = c->ny, nz = c->nz;
   for (int v = 0; v < c->nv; ++v) {
@@ -958,7 +1088,7 @@ static void zero_planes(orc_ctx* c) {
   }
 }
 
-/* MP_QUIET_3D, RHS.f90:18-175 (FLOWS and DAMPING branches excluded). */
+/* MP_QUIET_3D, RHS.f90:18-175 (the FLOWS branch excluded). */
 static void rhs_quiet(orc_ctx* c) {
   const long N = c->N;
   const int bh = c->cfg.periodic ? 5 : 1;
@@ -1003,9 +1133,10 @@ static void rhs_quiet(orc_ctx* c) {
       s_p[i] = -c->c2rho0[i] * div - vz[i] * c->gradp0_z[i] - sp * p[i];
     }
   zero_planes(c);
+  damp_rhs(c);
 }
 
-/* MP_MHD_3D, RHS.f90:363-668 (FLOWS and DAMPING branches excluded). */
+/* MP_MHD_3D, RHS.f90:363-668 (the FLOWS branch excluded). */
 static void rhs_mhd(orc_ctx* c) {
   const long N = c->N;
   const int bh = c->cfg.periodic ? 5 : 1;
@@ -1121,6 +1252,7 @@ static void rhs_mhd(orc_ctx* c) {
       s[7 * N + i] = rbz;
     }
   zero_planes(c);
+  damp_rhs(c);
 }
 
 /* Stage values + RHS.  step.f90:27-37 then the RHS dispatch (step.f90:39-46). */

@@ -73,6 +73,13 @@ void orc_diag(const orc_ctx* c, double out[3]); /* norm_KE (cgs), min(rho+rho0),
  * hist (may be NULL) receives 3 doubles per iteration. */
 void orc_run(orc_ctx* c, int n0, int n1, double* hist);
 
+/* DAMPING: on != 0 computes damping_rates (INIT_DAMPING, initialize.f90:
+ * 1276-1336) and makes every RHS subtract the damping term of vx, vy, vz
+ * (RHS.f90:166-172, 659-665).  orc_damp_term: damp_velocity of one 3-D
+ * field (all_modules.f90:1802-1888) with the current rates. */
+int orc_set_damping(orc_ctx* c, int on);
+void orc_damp_term(const orc_ctx* c, const double* v, double* term);
+
 /* RHS of the current temp_step / psi stage values into scr (one call of
  * MP_QUIET_3D or MP_MHD_3D with the stage psi = base + beta*K). */
 void orc_rhs_stage(orc_ctx* c, int stage);

@@ -30,6 +30,7 @@ class RunConfig:
     use_pml: bool = True
     use_hpml: bool = False
     stabilise: bool = True
+    damping: bool = False               # DAMPING: FFT damping of the velocities (RHS.f90:166-172)
     # PML lines "N, RC, n, kappa, sp" per face, in FACES order
     npml: Tuple[int, int, int, int, int, int] = (0, 0, 0, 0, 10, 10)
     pmlc: Tuple[Tuple[float, float, float, float], ...] = ((9.2103, 2.0, 1.0, 0.1),) * 6

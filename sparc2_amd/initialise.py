@@ -347,6 +347,23 @@ def prepare(cfg: RunConfig, raw: Dict[str, np.ndarray]) -> SetupState:
                       deltat=deltat, betas=rk_betas(deltat), bg=bg, bg3d=bg3d)
 
 
+def damping_rates(st: SetupState) -> np.ndarray:
+    """damping_rates of INIT_DAMPING (initialize.f90:1286-1314): per
+    horizontal wavenumber (kx = 0..nx/2, ky folded to |ky|), in C order
+    (ny, nx/2+1) = the Fortran array (nx/2+1, ny).  The factor 1/(nx*ny)
+    normalises the unnormalised R2C/C2R transform pair of damp_velocity."""
+    cfg = st.cfg
+    nx, ny = cfg.nx, cfg.ny
+    constx = 2.0 * PI * RSUN / cfg.xlength
+    consty = 2.0 * PI * RSUN / cfg.ylength
+    ii = np.arange(nx // 2 + 1, dtype=np.float64)
+    jj = np.arange(ny, dtype=np.float64)
+    ky = np.where(np.arange(ny) <= ny // 2, jj, ny - jj)
+    kay2d = (constx ** 2 * ii[None, :] ** 2 + consty ** 2 * ky[:, None] ** 2) ** 0.5
+    kayref = 902.0
+    return st.diml / st.dimc / nx / ny * 1.0e-6 * (500.0 * (kay2d / kayref) ** 2.2) * 20.0
+
+
 def source_profiles(st: SetupState):
     """Separable factors of forcing_p (driver.f90:143-145)."""
     cfg, diml = st.cfg, st.diml

@@ -312,7 +312,7 @@ def to_run_config(p: Dict, precision: str = "fp64") -> RunConfig:
         nx=p["nx"], ny=p["ny"], nz=p["nz"], xlength=p["xlength"], ylength=p["ylength"],
         timestep=p["timestep"], solartime=p["solartime"], magnetic=magnetic,
         periodic=p["PERIODIC"], use_pml=p["USE_PML"], use_hpml=p["USE_HPML"],
-        stabilise=p["STABILISE"], npml=npml, pmlc=pmlc,
+        stabilise=p["STABILISE"], damping=p["DAMPING"], npml=npml, pmlc=pmlc,
         filter_mode_xy=1 if p["g_xy_filt"] == 5 else 2,
         filter_mode_z=1 if p["g_z_filt"] == 5 else 2,
         s_xy=p["s_xy"], s_z=p["s_z"], t_xy=p["t_xy"], t_z=p["t_z"],
