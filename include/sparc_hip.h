@@ -93,6 +93,15 @@ int sparc_hip_destroy(sparc_ctx* ctx);
 /* Upload the post-set-up backgrounds (replaces the module arrays STEP reads). */
 int sparc_hip_set_background(sparc_ctx* ctx, const sparc_bg* bg);
 
+/* FFT damping of the velocities (DAMPING = 1): rates is damping_rates(nx/2+1,
+ * ny), Fortran order, as INIT_DAMPING sets it (initialize.f90:1276-1336; the
+ * rates carry the 1/(nx*ny) of the unnormalised transform pair).  Every RK
+ * stage then subtracts damp_velocity(v) -- per z plane the c2r transform of
+ * the r2c transform of vx, vy, vz times the rates (all_modules.f90:1802-1888)
+ * -- from their right-hand sides after the boundary zeroing (RHS.f90:166-172,
+ * 659-665).  NULL switches damping off.  Runs with damping are not batched. */
+int sparc_hip_set_damping(sparc_ctx* ctx, const double* rates);
+
 /* State in / out: a is nv*nx*ny*nz (field-major, the reference's `a`,
  * all_modules.f90:104); psi[f] is npsi*face_size for each active face (the
  * PML bases zpmlvars/xpmlvars/ypmlvars), NULL for inactive faces or to leave

@@ -22,7 +22,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libsparc_hip.so")
 H5LIB = os.path.join(HERE, "libsparc_h5.so")
 HDF5_DIR = os.environ.get("SPARC_HDF5_DIR", "/opt/conda")
-SOURCES = ["api.hip", "stage.hip", "ops.hip"]
+SOURCES = ["api.hip", "stage.hip", "ops.hip", "damp.hip"]
 ARCH = os.environ.get("SPARC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -73,7 +73,7 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-lhipfft", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

@@ -47,6 +47,11 @@ hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t 
 hipError_t launch_alfven_max(const double* red, const double* b2, const double* rho0, long n,
                              double dimb2, double T, double den0, double* part, int nblk,
                              hipStream_t st);
+struct DampPlan;
+DampPlan* damp_create(int nx, int ny, int nz, long fs, bool f32, const double* rates, hipStream_t st);
+void damp_destroy(DampPlan* d);
+hipError_t damp_terms(DampPlan* d, const void* u, hipStream_t st);
+const void* damp_term_slab(const DampPlan* d, int f);
 }  // namespace sparc
 
 using namespace sparc;
@@ -112,6 +117,7 @@ struct sparc_ctx {
   void *g = nullptr, *stretch = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
   double* stretch_d = nullptr;  // FP64 copy for the standalone operators
   FaceDev face[6];
+  DampPlan* damp = nullptr;  // FFT damping (DAMPING = 1), null when off
   double* partial = nullptr;
   long partial_cap = 0;  // blocks the partial buffer holds
   double* diag = nullptr;
@@ -186,6 +192,8 @@ static void free_all(sparc_ctx* c) {
   F(c->partial);
   F(c->diag);
   F(c->op_buf);
+  damp_destroy(c->damp);
+  c->damp = nullptr;
   for (auto& f : c->face) {
     F(f.a);
     F(f.b);
@@ -470,6 +478,20 @@ int sparc_hip_set_background(sparc_ctx* c, const sparc_bg* bg) {
   return SPARC_OK;
 }
 
+int sparc_hip_set_damping(sparc_ctx* c, const double* rates) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  if (c->damp) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    damp_destroy(c->damp);
+    c->damp = nullptr;
+  }
+  if (!rates) return SPARC_OK;
+  c->damp = damp_create(c->nx, c->ny, c->nz, c->fs, c->f32, rates, c->st);
+  if (!c->damp) return fail(SPARC_E_HIP, "FFT damping: plans or buffers could not be made");
+  return SPARC_OK;
+}
+
 int sparc_hip_put_state(sparc_ctx* c, const double* a, const double* const psi[6]) {
   if (!c) return fail(SPARC_E_INVALID, "null context");
   HIPCHK(hipSetDevice(c->dev));
@@ -604,6 +626,7 @@ static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
     A.a[f] = (T*)c->pool[c->ra] + (size_t)f * c->fs;
   }
   for (int f = 0; f < 6; ++f) A.dz[f] = c->dzbuf ? (T*)c->dzbuf + (size_t)f * c->fs : nullptr;
+  for (int f = 0; f < 3; ++f) A.damp[f] = c->damp ? (const T*)damp_term_slab(c->damp, f) : nullptr;
 }
 
 // Stage s of LDDRK-5: u_1 = a; stage s reads u_s and writes
@@ -629,6 +652,11 @@ static int do_step(sparc_ctx* c, double amp) {
   for (int s = 1; s <= 5; ++s) {
     set_stage<T>(c, A, s);
     TimeRec r;
+    if (c->damp) {
+      // damp_velocity of this stage's vx, vy, vz (RHS.f90:166-172)
+      const hipError_t e = damp_terms(c->damp, A.u[0], c->st);
+      if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("damping transforms: ") + hipGetErrorString(e));
+    }
     if (c->zp) {
       timed_begin(c, 4, &r);
       hipError_t e = launch_zpass<T>(A, c->st);
@@ -1039,6 +1067,8 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
       return fail(SPARC_E_INVALID, "batched runs must share device and precision");
     for (int q = 0; q < r; ++q)
       if (runs[q] == runs[r]) return fail(SPARC_E_INVALID, "a run appears twice in the batch");
+    if (runs[r]->damp)
+      return fail(SPARC_E_INVALID, "runs with FFT damping are advanced alone, not batched");
   }
   sparc_batch* b = new sparc_batch();
   b->dev = runs[0]->dev;

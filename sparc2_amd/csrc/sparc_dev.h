@@ -115,6 +115,7 @@ template <typename T> struct StageArgs {
   T* un[8];
   T* comp[8];  // Kahan compensation (fp32 only), updated in place on the last stage
   T* dz[6];    // MHD z-pass output: z derivatives in CPML family order
+  const T* damp[3];  // FFT damping terms of vx, vy, vz (null: no damping)
   // backgrounds: 3-D (bg3d) or nz-long z profiles
   const T *rho0, *rhoinv, *c2rho0, *gr0z, *gp0z, *sponge, *p0;
   // MHD backgrounds (always 3-D)

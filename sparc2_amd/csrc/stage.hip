@@ -640,6 +640,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           for (int f = 0; f < NV; ++f) set_lane_pt(K[f], c, T(0));
         }
       }
+      // FFT damping (RHS.f90:166-172, 659-665): after the zeroing, RHSv -=
+      // damp_velocity(v), the terms computed from this stage's u beforehand
+      if (Ap->damp[0]) {
+#pragma unroll
+        for (int f = 0; f < 3; ++f) K[VX + f] = K[VX + f] - ldv<V>(Buf<T>(Ap->damp[f], fs), g * W);
+      }
       if (!last) {
         // next stage value, step.f90:33
 #pragma unroll

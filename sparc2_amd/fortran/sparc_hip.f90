@@ -55,7 +55,8 @@ module sparc_hip
        sparc_hip_timing, sparc_hip_kernel_time, sparc_hip_tile, sparc_hip_version, &
        sparc_hip_alfven_search, sparc_hip_get_slab, sparc_hip_batch_create, &
        sparc_hip_batch_advance, sparc_hip_batch_info, sparc_hip_batch_destroy, &
-       sparc_hip_autotune, sparc_hip_set_tile, sparc_hip_batch_time_stage, sparc_error_message
+       sparc_hip_autotune, sparc_hip_set_tile, sparc_hip_batch_time_stage, sparc_hip_set_damping, &
+       sparc_error_message
 
   interface
     integer(c_int) function sparc_hip_create(cfg, ctx) bind(C, name="sparc_hip_create")
@@ -73,6 +74,14 @@ module sparc_hip
       import :: c_int, c_ptr, sparc_bg
       type(c_ptr), value :: ctx
       type(sparc_bg), intent(in) :: bg
+    end function
+
+    ! rates: c_loc of damping_rates(nx/2+1, ny) (INIT_DAMPING), or c_null_ptr
+    ! to switch the FFT damping off
+    integer(c_int) function sparc_hip_set_damping(ctx, rates) bind(C, name="sparc_hip_set_damping")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: rates
     end function
 
     ! a: the reference's a(dim1,dim2,dim3,dimfive); psi(f): c_loc of the PML

@@ -23,7 +23,7 @@ import numpy as np
 from . import _lib
 from ._lib import SparcBg, SparcCfg, check, dptr, lib
 from .config import RunConfig
-from .initialise import (SetupState, initialise_rhs, layout_for_device, prepare,
+from .initialise import (SetupState, damping_rates, initialise_rhs, layout_for_device, prepare,
                          source_amplitude, source_profiles)
 
 
@@ -99,6 +99,10 @@ class Solver:
             b.src_x, b.src_y, b.src_z = dptr(sx), dptr(sy), dptr(sz)
         b.dimc, b.dimrho = st.dimc, st.dimrho
         check(lib().sparc_hip_set_background(self.h, C.byref(b)), "sparc_hip_set_background")
+        if self.cfg.damping:
+            # DAMPING: INIT_DAMPING's rates, C (ny, nx/2+1) = Fortran (nx/2+1, ny)
+            self._damp_rates = np.ascontiguousarray(damping_rates(st), dtype=np.float64)
+            check(lib().sparc_hip_set_damping(self.h, dptr(self._damp_rates)), "sparc_hip_set_damping")
 
     def close(self):
         h = getattr(self, "h", None)
