@@ -102,6 +102,16 @@ int sparc_hip_set_background(sparc_ctx* ctx, const sparc_bg* bg);
  * 659-665).  NULL switches damping off.  Runs with damping are not batched. */
 int sparc_hip_set_damping(sparc_ctx* ctx, const double* rates);
 
+/* PULSE = 0 driver (driver.f90:73-80, 130-133): map is this step's
+ * forcing(nx, ny), Fortran order, nondimensional -- LAGRANGE_INTERP of the
+ * forcing cube times source_damp (all_modules.f90:1892-1950) -- and
+ * source_dep the depth profile (nz values, all_modules.f90:289-301; NULL keeps
+ * the last one).  Every RK stage then adds forcing*source_dep to RHSv_z
+ * (RHS.f90:146-147, 640-641) in place of the pulse.  Set before each step
+ * (the call waits for the context's queued work); map NULL returns to the
+ * pulse.  Runs driven this way are not batched. */
+int sparc_hip_set_forcing(sparc_ctx* ctx, const double* map, const double* source_dep);
+
 /* State in / out: a is nv*nx*ny*nz (field-major, the reference's `a`,
  * all_modules.f90:104); psi[f] is npsi*face_size for each active face (the
  * PML bases zpmlvars/xpmlvars/ypmlvars), NULL for inactive faces or to leave

@@ -84,6 +84,8 @@ def lib():
         L.orc_set_damping.argtypes = [vp, C.c_int]
         L.orc_set_damping.restype = C.c_int
         L.orc_damp_term.argtypes = [vp, P, P]
+        L.orc_set_forcing_map.argtypes = [vp, P, P]
+        L.orc_set_forcing_map.restype = C.c_int
         for n in ("orc_ddx", "orc_ddy", "orc_ddz"):
             getattr(L, n).argtypes = [vp, P, P, C.c_int]
             getattr(L, n).restype = C.c_int
@@ -175,6 +177,18 @@ class Oracle:
         out = np.zeros_like(v)
         lib().orc_damp_term(self.h, _p(v), _p(out))
         return out
+
+    # -- PULSE = 0 --
+    def set_forcing_map(self, fmap, sdep=None):
+        """This step's forcing (ny, nx) and source_dep (nz); None: the pulse."""
+        if fmap is None:
+            lib().orc_set_forcing_map(self.h, None, None)
+            return
+        self._fmap = np.ascontiguousarray(fmap, dtype=np.float64)
+        self._sdep = None if sdep is None else np.ascontiguousarray(sdep, dtype=np.float64)
+        if lib().orc_set_forcing_map(self.h, _p(self._fmap),
+                                     None if self._sdep is None else _p(self._sdep)) != 0:
+            raise MemoryError("forcing map")
 
     # -- set-up state --
     def scalar(self, name):

@@ -149,6 +149,9 @@ struct orc_ctx {
   double *a, *u, *scr, *forcing;
   double *w[NWORK];
   double* damp_rates; /* (ny, nx/2+1) when DAMPING is on, else NULL */
+  /* PULSE = 0: this step's forcing (nx*ny, LAGRANGE_INTERP x source_damp)
+   * and source_dep (nz); fmap NULL: the pulse */
+  double *fmap, *sdep;
 };
 
 #define IX(c, i, j, k) ((long)(i) + (long)(c)->nx * ((long)(j) + (long)(c)->ny * (long)(k)))
@@ -779,6 +782,10 @@ orc_ctx* orc_create(const orc_cfg* cfg, const double* rho0_in, const double* p0_
 }
 
 void orc_destroy(orc_ctx* c) {
+  if (c) {
+    free(c->fmap);
+    free(c->sdep);
+  }
   if (c) free(c->damp_rates);
   if (!c) return;
   double* ptrs[] = {c->xaxis, c->yaxis, c->zaxis, c->g, c->cq, c->stretch, c->unstretch,
@@ -1051,6 +1058,23 @@ void orc_damp_term(const orc_ctx* c, const double* v, double* term) {
   free(sy);
 }
 
+/* PULSE = 0 (RHS.f90:146-147, 640-641): RHSv_z += forcing * source_dep(kk),
+ * forcing = LAGRANGE_INTERP x source_damp of this step (driver.f90:130-132),
+ * given by the caller.  map NULL returns to the pulse. */
+int orc_set_forcing_map(orc_ctx* c, const double* map, const double* sdep) {
+  if (!map) {
+    free(c->fmap);
+    c->fmap = NULL;
+    return 0;
+  }
+  if (!c->fmap) c->fmap = dalloc((long)c->nx * c->ny);
+  if (!c->sdep) c->sdep = dalloc(c->nz);
+  if (!c->fmap || !c->sdep) return -1;
+  memcpy(c->fmap, map, sizeof(double) * c->nx * c->ny);
+  if (sdep) memcpy(c->sdep, sdep, sizeof(double) * c->nz);
+  return 0;
+}
+
 /* RHSv -= damp_velocity(v) for the three velocities of the stage value. */
 static void damp_rhs(orc_ctx* c) {
   if (!c->damp_rates) return;
@@ -1130,6 +1154,7 @@ static void rhs_quiet(orc_ctx* c) {
       if (pd == 1) s_vx[i] = s_vx[i] + c->forcing[i];
       if (pd == 2) s_vy[i] = s_vy[i] + c->forcing[i];
       if (pd == 3) s_vz[i] = s_vz[i] + c->forcing[i];
+      if (c->fmap) s_vz[i] = s_vz[i] + c->fmap[q] * c->sdep[k];
       s_p[i] = -c->c2rho0[i] * div - vz[i] * c->gradp0_z[i] - sp * p[i];
     }
   zero_planes(c);
@@ -1242,6 +1267,7 @@ static void rhs_mhd(orc_ctx* c) {
       if (pd == 1) rvx = rvx + c->forcing[i];
       if (pd == 2) rvy = rvy + c->forcing[i];
       if (pd == 3) rvz = rvz + c->forcing[i];
+      if (c->fmap) rvz = rvz + c->fmap[q] * c->sdep[k];
       s[i] = rc;
       s[N + i] = rvx;
       s[2 * N + i] = rvy;

@@ -78,6 +78,10 @@ void orc_run(orc_ctx* c, int n0, int n1, double* hist);
  * (RHS.f90:166-172, 659-665).  orc_damp_term: damp_velocity of one 3-D
  * field (all_modules.f90:1802-1888) with the current rates. */
 int orc_set_damping(orc_ctx* c, int on);
+/* PULSE = 0: this step's forcing(nx, ny) (LAGRANGE_INTERP x source_damp) and
+ * source_dep(nz) (NULL keeps it); the RHS adds forcing*source_dep to RHSv_z
+ * (RHS.f90:146-147, 640-641).  map NULL returns to the pulse. */
+int orc_set_forcing_map(orc_ctx* c, const double* map, const double* sdep);
 void orc_damp_term(const orc_ctx* c, const double* v, double* term);
 
 /* RHS of the current temp_step / psi stage values into scr (one call of

@@ -51,6 +51,7 @@ class SparcBg(C.Structure):
 
 EXPORTS = [
     "sparc_hip_create", "sparc_hip_destroy", "sparc_hip_set_background", "sparc_hip_set_damping",
+    "sparc_hip_set_forcing",
     "sparc_hip_put_state",
     "sparc_hip_get_state", "sparc_hip_step", "sparc_hip_filter", "sparc_hip_advance",
     "sparc_hip_diag", "sparc_hip_ddx", "sparc_hip_ddy", "sparc_hip_ddz", "sparc_hip_filtx",
@@ -83,6 +84,7 @@ def lib() -> C.CDLL:
     L.sparc_hip_destroy.argtypes = [vp]
     L.sparc_hip_set_background.argtypes = [vp, C.POINTER(SparcBg)]
     L.sparc_hip_set_damping.argtypes = [vp, DP]
+    L.sparc_hip_set_forcing.argtypes = [vp, DP, DP]
     L.sparc_hip_put_state.argtypes = [vp, DP, DP * 6]
     L.sparc_hip_get_state.argtypes = [vp, DP, DP * 6]
     L.sparc_hip_step.argtypes = [vp, C.c_double]

@@ -1,12 +1,14 @@
 """Build libsparc_hip.so (the HIP kernels + C ABI) in-tree for gfx950, and
 libsparc_h5.so (the reference's HDF5 file layouts, host code over the HDF5 C
-library) when HDF5 is installed.
+library) when HDF5 is installed, and libsparc_fits.so (the forcing cube of
+the PULSE = 0 driver, over CFITSIO) when CFITSIO is.
 
     python -m sparc2_amd.build [--jobs N]
 
 Each .hip source is compiled to an object with hipcc, then linked into
 sparc2_amd/libsparc_hip.so.  Packed FP32 instructions are disabled for all
-device code.  HDF5 is looked up under $SPARC_HDF5_DIR (default /opt/conda).
+device code.  HDF5 and CFITSIO are looked up under $SPARC_HDF5_DIR (default
+/opt/conda).
 """
 from __future__ import annotations
 
@@ -21,6 +23,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libsparc_hip.so")
 H5LIB = os.path.join(HERE, "libsparc_h5.so")
+FITSLIB = os.path.join(HERE, "libsparc_fits.so")
 HDF5_DIR = os.environ.get("SPARC_HDF5_DIR", "/opt/conda")
 SOURCES = ["api.hip", "stage.hip", "ops.hip", "damp.hip"]
 ARCH = os.environ.get("SPARC_OFFLOAD_ARCH", "gfx950")
@@ -68,6 +71,28 @@ def build_h5(verbose: bool = True):
     return H5LIB
 
 
+def build_fits(verbose: bool = True):
+    """libsparc_fits.so from csrc/fitsio.cpp with g++ (the forcing cube of the
+    PULSE = 0 driver); None when CFITSIO is absent."""
+    inc, libdir = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    if not (os.path.exists(os.path.join(inc, "fitsio.h")) and
+            os.path.exists(os.path.join(libdir, "libcfitsio.so"))):
+        if verbose:
+            print(f"CFITSIO not found under {HDF5_DIR}: libsparc_fits.so not built")
+        return None
+    tmp = FITSLIB + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", f"-I{inc}",
+           os.path.join(CSRC, "fitsio.cpp"), f"-L{libdir}", "-lcfitsio",
+           f"-Wl,-rpath,{libdir}", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for fitsio.cpp:\n{r.stderr}")
+    os.replace(tmp, FITSLIB)
+    if verbose:
+        print(f"built {FITSLIB}")
+    return FITSLIB
+
+
 def build(jobs: int = 4, verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
@@ -81,6 +106,7 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
     if verbose:
         print(f"built {LIB}")
     build_h5(verbose)
+    build_fits(verbose)
     return LIB
 
 

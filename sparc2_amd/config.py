@@ -41,6 +41,13 @@ class RunConfig:
     t_xy: float = 30.0                  # s
     t_z: float = 10.0                   # s
     pulse: bool = True
+    # PULSE = 0 driver (a forcing cube read from a FITS file, driver.f90:73-80,
+    # 130-133): file, cadence (s), depth (cm), sigma (cm), amplitude (cm/s)
+    forcingfunc: str = ""
+    cadforcing: float = 60.0
+    randomexcitedepth: float = -0.15e8
+    randomexcitesigma: float = 0.1e8
+    randomexciteampl: float = 1.0e4
     pulse_dir: int = 3
     pulse_t0: float = 1000.0 / 3.0      # s
     pulse_t1: float = 600.0

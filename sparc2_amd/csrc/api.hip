@@ -118,6 +118,9 @@ struct sparc_ctx {
   double* stretch_d = nullptr;  // FP64 copy for the standalone operators
   FaceDev face[6];
   DampPlan* damp = nullptr;  // FFT damping (DAMPING = 1), null when off
+  // PULSE = 0 driver: forcing map (nx*ny) and source_dep (nz), element type
+  void *fmap = nullptr, *sdep = nullptr;
+  bool forcing_on = false;
   double* partial = nullptr;
   long partial_cap = 0;  // blocks the partial buffer holds
   double* diag = nullptr;
@@ -192,6 +195,8 @@ static void free_all(sparc_ctx* c) {
   F(c->partial);
   F(c->diag);
   F(c->op_buf);
+  F(c->fmap);
+  F(c->sdep);
   damp_destroy(c->damp);
   c->damp = nullptr;
   for (auto& f : c->face) {
@@ -492,6 +497,26 @@ int sparc_hip_set_damping(sparc_ctx* c, const double* rates) {
   return SPARC_OK;
 }
 
+int sparc_hip_set_forcing(sparc_ctx* c, const double* map, const double* source_dep) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  if (!map) {
+    c->forcing_on = false;
+    return SPARC_OK;
+  }
+  int rc;
+  if (!c->fmap && (rc = dalloc(&c->fmap, c->esz * (size_t)c->nx * c->ny))) return rc;
+  if (!c->sdep && (rc = dalloc(&c->sdep, c->esz * (size_t)c->nz))) return rc;
+  if (source_dep) {
+    if ((rc = upload(c, c->sdep, source_dep, c->nz))) return rc;
+  } else if (!c->forcing_on) {
+    return fail(SPARC_E_INVALID, "the first forcing map needs source_dep");
+  }
+  if ((rc = upload(c, c->fmap, map, (long)c->nx * c->ny))) return rc;
+  c->forcing_on = true;
+  return SPARC_OK;
+}
+
 int sparc_hip_put_state(sparc_ctx* c, const double* a, const double* const psi[6]) {
   if (!c) return fail(SPARC_E_INVALID, "null context");
   HIPCHK(hipSetDevice(c->dev));
@@ -627,6 +652,8 @@ static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
   }
   for (int f = 0; f < 6; ++f) A.dz[f] = c->dzbuf ? (T*)c->dzbuf + (size_t)f * c->fs : nullptr;
   for (int f = 0; f < 3; ++f) A.damp[f] = c->damp ? (const T*)damp_term_slab(c->damp, f) : nullptr;
+  A.fmap = c->forcing_on ? (const T*)c->fmap : nullptr;
+  A.sdep = (const T*)c->sdep;
 }
 
 // Stage s of LDDRK-5: u_1 = a; stage s reads u_s and writes
@@ -1069,6 +1096,8 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
       if (runs[q] == runs[r]) return fail(SPARC_E_INVALID, "a run appears twice in the batch");
     if (runs[r]->damp)
       return fail(SPARC_E_INVALID, "runs with FFT damping are advanced alone, not batched");
+    if (runs[r]->forcing_on)
+      return fail(SPARC_E_INVALID, "runs driven by a forcing map are advanced alone, not batched");
   }
   sparc_batch* b = new sparc_batch();
   b->dev = runs[0]->dev;

@@ -116,6 +116,10 @@ template <typename T> struct StageArgs {
   T* comp[8];  // Kahan compensation (fp32 only), updated in place on the last stage
   T* dz[6];    // MHD z-pass output: z derivatives in CPML family order
   const T* damp[3];  // FFT damping terms of vx, vy, vz (null: no damping)
+  // PULSE = 0 driver: this step's forcing(x, y) (nx*ny) and source_dep(z);
+  // null fmap: the pulse (or no source)
+  const T* fmap;
+  const T* sdep;
   // backgrounds: 3-D (bg3d) or nz-long z profiles
   const T *rho0, *rhoinv, *c2rho0, *gr0z, *gp0z, *sponge, *p0;
   // MHD backgrounds (always 3-D)

@@ -621,9 +621,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         K[VY] = -rhoinv * (gpy - c2 * red) - sp * vy + rhoinv * e2 * red;
         K[VZ] = -rhoinv * (gpz - c3 * red) - sp * vz + rhoinv * e3 * red - rhoinv * rho * ((CPtr<T>)Ap->g)[k];
       }
-      // pulse source (driver.f90:137-148 separable form; RHS.f90:138-145,632-639)
+      // pulse source (driver.f90:137-148 separable form; RHS.f90:138-145,632-639),
+      // or with PULSE = 0 forcing(x, y) * source_dep(z) on vz (RHS.f90:146-147, 640-641)
       const int pdir = Ap->pulse_dir;
-      if (pdir >= 1 && pdir <= 3) {
+      if (Ap->fmap) {
+        K[VZ] += ldv<V>(Buf<T>(Ap->fmap, nxy), (uint32_t)(i + nx * j) * W) * ((CPtr<T>)Ap->sdep)[k];
+      } else if (pdir >= 1 && pdir <= 3) {
         const T amp = Ap->amp_tab ? Ap->amp_tab[*Ap->tstep] : Ap->src_amp;
         const V s = amp * ldv<V>(Buf<T>(Ap->sx, nx), (uint32_t)i * W) * Buf<T>(Ap->sy, ny)[j] *
                     ((CPtr<T>)Ap->sz)[k];

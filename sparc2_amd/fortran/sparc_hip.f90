@@ -56,7 +56,7 @@ module sparc_hip
        sparc_hip_alfven_search, sparc_hip_get_slab, sparc_hip_batch_create, &
        sparc_hip_batch_advance, sparc_hip_batch_info, sparc_hip_batch_destroy, &
        sparc_hip_autotune, sparc_hip_set_tile, sparc_hip_batch_time_stage, sparc_hip_set_damping, &
-       sparc_error_message
+       sparc_hip_set_forcing, sparc_error_message
 
   interface
     integer(c_int) function sparc_hip_create(cfg, ctx) bind(C, name="sparc_hip_create")
@@ -82,6 +82,16 @@ module sparc_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       type(c_ptr), value :: rates
+    end function
+
+    ! PULSE = 0: map = c_loc(forcing(nx,ny)) after LAGRANGE_INTERP and
+    ! source_damp, sdep = c_loc(source_dep(nz)) or c_null_ptr to keep it;
+    ! map = c_null_ptr returns to the pulse
+    integer(c_int) function sparc_hip_set_forcing(ctx, map, sdep) bind(C, name="sparc_hip_set_forcing")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: map
+      type(c_ptr), value :: sdep
     end function
 
     ! a: the reference's a(dim1,dim2,dim3,dimfive); psi(f): c_loc of the PML

@@ -67,6 +67,8 @@ class SetupState:
     betas: np.ndarray
     bg: Dict[str, np.ndarray]  # nondimensional 3-D/1-D arrays
     pml: Dict[int, Dict[str, np.ndarray]] = dataclasses.field(default_factory=dict)
+    # PULSE = 0 driver: source_damp(x, y) of initialize.f90:1202-1216, C (ny, nx)
+    source_damp: Optional[np.ndarray] = None
     bg3d: bool = True
 
 
@@ -271,6 +273,8 @@ def initialise_rhs(st: SetupState, ops) -> None:
             g[ks] = g[ks] * alpha / k11 / (d11 / k11 + alpha)
             sponge[ks] = sponge[ks] + sp * tsl[None, :, :] * (s ** (pn + 1.0))[:, None, None]
             del fac
+        # source_dampx / source_dampy: the x/y CPML profiles ((edge - x)/L)^n
+        sdx, sdy = np.zeros(nx), np.zeros(ny)
         if cfg.use_hpml:
             xa, ya = st.xaxis, st.yaxis
             for f in (0, 1):   # initialize.f90:1108-1152
@@ -292,6 +296,7 @@ def initialise_rhs(st: SetupState, ops) -> None:
                     alpha = f0 * (xa[nx - 1] - xa[iis]) / L
                     tsl = tin[:, :, i0]
                 sn = s ** pn
+                sdx[iis] = sn
                 damp = RC * tsl[:, :, None] * (pn + 1.0) / (2.0 * L) * sn[None, None, :]
                 kap = np.broadcast_to((1.0 + (kap_max - 1.0) * sn)[None, None, :], damp.shape)
                 al = alpha[None, None, :]
@@ -317,6 +322,7 @@ def initialise_rhs(st: SetupState, ops) -> None:
                     alpha = f0 * (ya[ny - 1] - ya[jjs]) / L
                     tsl = tin[:, j0, :]
                 sn = s ** pn
+                sdy[jjs] = sn
                 damp = RC * tsl[:, None, :] * (pn + 1.0) / (2.0 * L) * sn[None, :, None]
                 kap = np.broadcast_to((1.0 + (kap_max - 1.0) * sn)[None, :, None], damp.shape)
                 al = alpha[None, :, None]
@@ -324,6 +330,14 @@ def initialise_rhs(st: SetupState, ops) -> None:
                              "kappa": np.ascontiguousarray(kap)}
                 sponge[:, jjs, :] = sponge[:, jjs, :] + sp * tsl[:, None, :] * (s ** (pn + 1.0))[None, :, None]
 
+    # source_damp (initialize.f90:1202-1216): 1 when periodic, else one minus
+    # the larger x/y CPML profile (1 outside the x/y CPML, 1 without it)
+    if cfg.periodic:
+        st.source_damp = np.ones((ny, nx))
+    elif cfg.use_pml:
+        st.source_damp = 1.0 - np.maximum(sdx[None, :], sdy[:, None])
+    else:
+        st.source_damp = np.ones((ny, nx))
     bg["gradrho0_z"] = grz
     bg["gradp0_z"] = gpz
     bg["g"] = g

@@ -23,6 +23,7 @@ import numpy as np
 from . import _lib
 from ._lib import SparcBg, SparcCfg, check, dptr, lib
 from .config import RunConfig
+from .forcing import ForcingSeries, source_dep
 from .initialise import (SetupState, damping_rates, initialise_rhs, layout_for_device, prepare,
                          source_amplitude, source_profiles)
 
@@ -78,6 +79,12 @@ class Solver:
             initialise_rhs(st, self)
         self._upload_background()
         self.put_state(np.zeros((self.nv,) + self.shape))
+        # PULSE = 0: the forcing series of the FITS cube (use_forcing attaches
+        # another one); every step uploads its forcing map first
+        self.series: Optional[ForcingSeries] = None
+        self._sdep_sent = False
+        if not cfg.pulse and cfg.forcingfunc:
+            self.use_forcing(ForcingSeries.from_file(st))
 
     # ---------------------------------------------------------------- set-up
     def _upload_background(self):
@@ -151,13 +158,39 @@ class Solver:
     def amplitude(self, time: int) -> float:
         return source_amplitude(self.st, time)
 
+    def use_forcing(self, series: Optional[ForcingSeries]):
+        """Drive the run with a PULSE = 0 forcing series (None: back to the
+        pulse settings of the run config)."""
+        self.series = series
+        self._sdep_sent = False
+        if series is None:
+            check(lib().sparc_hip_set_forcing(self.h, None, None), "sparc_hip_set_forcing")
+
+    def _push_forcing(self, time: int):
+        fmap = np.ascontiguousarray(self.series.forcing(time), dtype=np.float64)
+        sdep = None
+        if not self._sdep_sent:
+            self._sdep = np.ascontiguousarray(source_dep(self.st), dtype=np.float64)
+            sdep = dptr(self._sdep)
+        check(lib().sparc_hip_set_forcing(self.h, dptr(fmap), sdep), "sparc_hip_set_forcing")
+        self._sdep_sent = True
+
     def step(self, time: int):
+        if self.series is not None:
+            self._push_forcing(time)
         check(lib().sparc_hip_step(self.h, self.amplitude(time)), "sparc_hip_step")
 
     def filter_vars(self, time: int):
         check(lib().sparc_hip_filter(self.h, int(time)), "sparc_hip_filter")
 
     def advance(self, n0: int, n1: int):
+        if self.series is not None:
+            # the forcing map changes every step: the driver loop runs here
+            for t in range(n0, n1 + 1):
+                self.step(t)
+                if t != 0:
+                    self.filter_vars(t)
+            return
         amps = np.array([self.amplitude(t) for t in range(n0, n1 + 1)], dtype=np.float64)
         check(lib().sparc_hip_advance(self.h, int(n0), int(n1), dptr(amps)), "sparc_hip_advance")
 

@@ -295,14 +295,15 @@ def _finish(p: Dict) -> None:
 
 
 def to_run_config(p: Dict, precision: str = "fp64") -> RunConfig:
-    """The RunConfig of a parsed control file, for the 3-D pulse-driven
-    velocity-form runs this package implements."""
+    """The RunConfig of a parsed control file, for the 3-D velocity-form runs
+    this package implements: the single pulse (PULSE = 1) or the forcing cube
+    of a FITS file (PULSE = 0)."""
     if p["TWOD"]:
         raise ParameterError("2-D runs (RHS2D.f90) are not part of this package")
     if p["DISPL"] or p["FLOWS"]:
         raise ParameterError("displacement form and flows are not part of this package")
-    if not p["PULSE"] or p["PSRPULSE"]:
-        raise ParameterError("only the single-pulse driver (PULSE = 1) is implemented")
+    if p["PSRPULSE"]:
+        raise ParameterError("the PSR pulse driver (PULSE = 2) is not part of this package")
     if p["g_xy_filt"] == 0 or p["g_z_filt"] == 0:
         raise ParameterError("filter_mode_xy and filter_mode_z must be 1 or 2")
     magnetic = 2 if p["MAGNETIC"] else (0 if p["QUIET"] else 1)
@@ -316,7 +317,10 @@ def to_run_config(p: Dict, precision: str = "fp64") -> RunConfig:
         filter_mode_xy=1 if p["g_xy_filt"] == 5 else 2,
         filter_mode_z=1 if p["g_z_filt"] == 5 else 2,
         s_xy=p["s_xy"], s_z=p["s_z"], t_xy=p["t_xy"], t_z=p["t_z"],
-        pulse=True, pulse_dir=p["pulse_dir"], pulse_t0=p["pulse_t0"], pulse_t1=p["pulse_t1"],
+        pulse=bool(p["PULSE"]), forcingfunc=str(p.get("forcingfunc") or ""),
+        cadforcing=p["cadforcing"], randomexcitedepth=p["randomexcitedepth"],
+        randomexcitesigma=p["randomexcitesigma"], randomexciteampl=p["randomexciteampl"],
+        pulse_dir=p["pulse_dir"], pulse_t0=p["pulse_t0"], pulse_t1=p["pulse_t1"],
         pulse_st=p["pulse_st"], pulse_px=p["pulse_px"], pulse_py=p["pulse_py"],
         pulse_pz=p["pulse_pz"], pulse_sr=p["pulse_sr"], pulse_sz=p["pulse_sz"],
         pulse_amp=p["pulse_amp"], sp_str=p["sp_str"], sp_length=p["sp_length"],

@@ -128,6 +128,18 @@ def test_h5_library_exports_every_declared_entry_point():
         assert hasattr(L, name), name
 
 
+def test_fits_library_exports_every_declared_entry_point():
+    from sparc2_amd import forcing
+    if not os.path.exists(forcing.LIB):
+        pytest.skip("libsparc_fits.so not built (no CFITSIO)")
+    hdr = open(os.path.join(ROOT, "include", "sparc_fits.h")).read()
+    declared = sorted(set(re.findall(r"^(?:int|const char\*)\s+(sparc_fits_\w+)\(", hdr, re.M)))
+    assert len(declared) == 3
+    L = ctypes.CDLL(forcing.LIB)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
 def test_library_refuses_to_run_without_a_device():
     """No silent CPU fallback: creating a context without a HIP device fails."""
     if not os.path.exists(_lib.LIB_PATH):
