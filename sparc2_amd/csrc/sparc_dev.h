@@ -222,23 +222,29 @@ __device__ __forceinline__ void set_lane_pt(F2& v, int c, float s) {
 }
 
 // Lane-value load / store through a buffer view of T elements: byte offset
-// bo (Buf::kOut reads 0) or element index e of the lane's first point.
-template <typename V, typename T>
+// bo (Buf::kOut reads 0) or element index e of the lane's first point.  CP
+// is the cache policy: 0, or kStream (gfx950 `nt`, non-temporal) for values
+// that no other block of the launch touches -- the point values, the RK base
+// and the outputs -- so they do not evict the staged planes whose halos the
+// neighbouring tiles read again from L2.
+constexpr int kStream = 2;
+template <typename V, int CP = 0, typename T>
 __device__ __forceinline__ V ldv(const Buf<T>& b, uint32_t bo) {
-  if constexpr (sizeof(V) == sizeof(T)) {
-    return b.ldb(bo);
+  if constexpr (sizeof(V) == 8) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)bo, 0, CP));
   } else {
-    static_assert(sizeof(V) == 8 && sizeof(T) == 4, "two FP32 points per lane");
-    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)bo, 0, 0));
+    static_assert(sizeof(V) == 4, "a lane value of 4 or 8 bytes");
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)bo, 0, CP));
   }
 }
-template <typename V, typename T>
+template <int CP = 0, typename V, typename T>
 __device__ __forceinline__ void stv(const Buf<T>& b, uint32_t e, V v) {
-  if constexpr (sizeof(V) == sizeof(T)) {
-    b.st(e, v);
-  } else {
+  if constexpr (sizeof(V) == 8) {
     using R = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(R, v), b.r, (int)(e * 4u), 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(R, v), b.r, (int)(e * (uint32_t)sizeof(T)), 0, CP);
+  } else {
+    static_assert(sizeof(V) == 4, "a lane value of 4 or 8 bytes");
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), b.r, (int)(e * 4u), 0, CP);
   }
 }
 
@@ -250,28 +256,28 @@ __device__ __forceinline__ void stv(const Buf<T>& b, uint32_t e, V v) {
 // family fam[i]; fp the face element of the lane's first point.  All loads
 // of the face point (a, b, 1/kappa once, then every family's base and stage
 // value) are issued before any store, so they share one memory wait.  FR is
-// a Face<T> in any address space.
-template <typename T, typename V, int NF, typename FR>
+// a Face<T> in any address space; CP the cache policy of its accesses.
+template <int CP = 0, typename T, typename V, int NF, typename FR>
 __device__ __forceinline__ void pml_face(FR& F, uint32_t fp, V (&v)[NF], const int (&fam)[NF],
                                          bool first, bool last, T beta_next, T dt) {
   const uint32_t n = (uint32_t)F.fsize;
   const uint32_t bo = fp * (uint32_t)sizeof(T);
-  const V ca = ldv<V>(Buf<T>(F.a, n), bo), cb = ldv<V>(Buf<T>(F.b, n), bo),
-          ck = ldv<V>(Buf<T>(F.kinv, n), bo);
+  const V ca = ldv<V, CP>(Buf<T>(F.a, n), bo), cb = ldv<V, CP>(Buf<T>(F.b, n), bo),
+          ck = ldv<V, CP>(Buf<T>(F.kinv, n), bo);
   V base[NF], ps[NF];
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
-    base[i] = ldv<V>(Buf<T>(F.psi + (size_t)fam[i] * n, n), bo);
-    ps[i] = first ? base[i] : ldv<V>(Buf<T>(F.pst + (size_t)fam[i] * n, n), bo);
+    base[i] = ldv<V, CP>(Buf<T>(F.psi + (size_t)fam[i] * n, n), bo);
+    ps[i] = first ? base[i] : ldv<V, CP>(Buf<T>(F.pst + (size_t)fam[i] * n, n), bo);
   }
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
     const V kn = ca * v[i] + cb * ps[i];
     v[i] = v[i] * ck + ps[i];
     if (last)
-      stv(Buf<T>(F.psi + (size_t)fam[i] * n, n), fp, base[i] + dt * kn);
+      stv<CP>(Buf<T>(F.psi + (size_t)fam[i] * n, n), fp, base[i] + dt * kn);
     else
-      stv(Buf<T>(F.pst + (size_t)fam[i] * n, n), fp, base[i] + beta_next * kn);
+      stv<CP>(Buf<T>(F.pst + (size_t)fam[i] * n, n), fp, base[i] + beta_next * kn);
   }
 }
 
