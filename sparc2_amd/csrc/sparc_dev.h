@@ -256,28 +256,29 @@ __device__ __forceinline__ void stv(const Buf<T>& b, uint32_t e, V v) {
 // family fam[i]; fp the face element of the lane's first point.  All loads
 // of the face point (a, b, 1/kappa once, then every family's base and stage
 // value) are issued before any store, so they share one memory wait.  FR is
-// a Face<T> in any address space; CP the cache policy of its accesses.
-template <int CP = 0, typename T, typename V, int NF, typename FR>
+// a Face<T> in any address space; CPL and CPW the cache policies of its
+// loads and stores.
+template <int CPL = 0, int CPW = CPL, typename T, typename V, int NF, typename FR>
 __device__ __forceinline__ void pml_face(FR& F, uint32_t fp, V (&v)[NF], const int (&fam)[NF],
                                          bool first, bool last, T beta_next, T dt) {
   const uint32_t n = (uint32_t)F.fsize;
   const uint32_t bo = fp * (uint32_t)sizeof(T);
-  const V ca = ldv<V, CP>(Buf<T>(F.a, n), bo), cb = ldv<V, CP>(Buf<T>(F.b, n), bo),
-          ck = ldv<V, CP>(Buf<T>(F.kinv, n), bo);
+  const V ca = ldv<V, CPL>(Buf<T>(F.a, n), bo), cb = ldv<V, CPL>(Buf<T>(F.b, n), bo),
+          ck = ldv<V, CPL>(Buf<T>(F.kinv, n), bo);
   V base[NF], ps[NF];
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
-    base[i] = ldv<V, CP>(Buf<T>(F.psi + (size_t)fam[i] * n, n), bo);
-    ps[i] = first ? base[i] : ldv<V, CP>(Buf<T>(F.pst + (size_t)fam[i] * n, n), bo);
+    base[i] = ldv<V, CPL>(Buf<T>(F.psi + (size_t)fam[i] * n, n), bo);
+    ps[i] = first ? base[i] : ldv<V, CPL>(Buf<T>(F.pst + (size_t)fam[i] * n, n), bo);
   }
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
     const V kn = ca * v[i] + cb * ps[i];
     v[i] = v[i] * ck + ps[i];
     if (last)
-      stv<CP>(Buf<T>(F.psi + (size_t)fam[i] * n, n), fp, base[i] + dt * kn);
+      stv<CPW>(Buf<T>(F.psi + (size_t)fam[i] * n, n), fp, base[i] + dt * kn);
     else
-      stv<CP>(Buf<T>(F.pst + (size_t)fam[i] * n, n), fp, base[i] + beta_next * kn);
+      stv<CPW>(Buf<T>(F.pst + (size_t)fam[i] * n, n), fp, base[i] + beta_next * kn);
   }
 }
 

@@ -138,10 +138,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   // the outputs, the CPML arrays) take the non-temporal policy, so the L2
   // keeps the staged planes for the halo reads of the neighbouring tiles:
   // MHD 256^3 FP64 xy kernel 1.93 -> 1.85 ms, quiet 512x512x300 2.69 ->
-  // 2.60 ms (FP64), 1.46 -> 1.41 ms (FP32).  The FP32 MHD xy kernel ran
-  // slower with it (1.10 -> 1.17 ms) and keeps the default policy.
-  constexpr int CPS = (MHD && sizeof(T) == 4) ? 0 : kStream;
-  auto LDS1 = [&](const T* p, uint32_t bo) -> V { return ldv<V, CPS>(Buf<T>(p, fs), bo); };
+  // 2.60 ms (FP64), 1.46 -> 1.41 ms (FP32).  The FP32 MHD xy kernel gains
+  // from it on its point loads (1.09 -> 1.06 ms) but loses on its stores
+  // (-> 1.19 ms) and on the CPML arrays (-> 1.13 ms), so those keep the
+  // default policy there (CPW: stores and CPML).
+  constexpr int CPL = kStream, CPW = (MHD && sizeof(T) == 4) ? 0 : kStream;
+  auto LDS1 = [&](const T* p, uint32_t bo) -> V { return ldv<V, CPL>(Buf<T>(p, fs), bo); };
   auto slot_bo = [&](int s, uint32_t pk) -> uint32_t {
     return s_ok[s] ? (pk + s_off[s]) * W : Buf<T>::kOut;
   };
@@ -504,12 +506,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         if constexpr (MHD) {
           V v[6] = {gpz, dvzdz, dzf2, dzf1, dzbx, dzby};
           constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
-          pml_face<CPS>(F, fp, v, fam, first, last, bnext, dt);
+          pml_face<CPW>(F, fp, v, fam, first, last, bnext, dt);
           gpz = v[0], dvzdz = v[1], dzf2 = v[2], dzf1 = v[3], dzbx = v[4], dzby = v[5];
         } else {
           V v[2] = {gpz, dvzdz};
           constexpr int fam[2] = {1, 0};
-          pml_face<CPS>(F, fp, v, fam, first, last, bnext, dt);
+          pml_face<CPW>(F, fp, v, fam, first, last, bnext, dt);
           gpz = v[0], dvzdz = v[1];
         }
       }
@@ -530,12 +532,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
               if constexpr (MHD) {
                 V v[6] = {gpx, dvxdx, dxf3, dxf2, dxby, dxbz};
                 constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
-                pml_face<CPS>(F, fp, v, fam, first, last, bnext, dt);
+                pml_face<CPW>(F, fp, v, fam, first, last, bnext, dt);
                 gpx = v[0], dvxdx = v[1], dxf3 = v[2], dxf2 = v[3], dxby = v[4], dxbz = v[5];
               } else {
                 V v[2] = {gpx, dvxdx};
                 constexpr int fam[2] = {1, 0};
-                pml_face<CPS>(F, fp, v, fam, first, last, bnext, dt);
+                pml_face<CPW>(F, fp, v, fam, first, last, bnext, dt);
                 gpx = v[0], dvxdx = v[1];
               }
             }
@@ -551,7 +553,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
             T v[6] = {lane_pt(gpx, c),  lane_pt(dvxdx, c), lane_pt(dxf3, c),
                       lane_pt(dxf2, c), lane_pt(dxby, c),  lane_pt(dxbz, c)};
             constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
-            pml_face<CPS>(F, fp, v, fam, first, last, bnext, dt);
+            pml_face<CPW>(F, fp, v, fam, first, last, bnext, dt);
             set_lane_pt(gpx, c, v[0]);
             set_lane_pt(dvxdx, c, v[1]);
             set_lane_pt(dxf3, c, v[2]);
@@ -561,7 +563,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           } else {
             T v[2] = {lane_pt(gpx, c), lane_pt(dvxdx, c)};
             constexpr int fam[2] = {1, 0};
-            pml_face<CPS>(F, fp, v, fam, first, last, bnext, dt);
+            pml_face<CPW>(F, fp, v, fam, first, last, bnext, dt);
             set_lane_pt(gpx, c, v[0]);
             set_lane_pt(dvxdx, c, v[1]);
           }
@@ -576,12 +578,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         if constexpr (MHD) {
           V v[6] = {gpy, dvydy, dyf3, dyf1, dybx, dybz};
           constexpr int fam[6] = {1, 0, 4, 5, 2, 3};
-          pml_face<CPS>(F, fp, v, fam, first, last, bnext, dt);
+          pml_face<CPW>(F, fp, v, fam, first, last, bnext, dt);
           gpy = v[0], dvydy = v[1], dyf3 = v[2], dyf1 = v[3], dybx = v[4], dybz = v[5];
         } else {
           V v[2] = {gpy, dvydy};
           constexpr int fam[2] = {1, 0};
-          pml_face<CPS>(F, fp, v, fam, first, last, bnext, dt);
+          pml_face<CPW>(F, fp, v, fam, first, last, bnext, dt);
           gpy = v[0], dvydy = v[1];
         }
       }
@@ -665,7 +667,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         for (int f = 0; f < NV; ++f) {
           const Buf<T> uf(Ap->un[f], fs);
           const V a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : ldv<V>(Buf<T>(Ap->a[f], fs), g * W));
-          stv<CPS>(uf, g, a0 + bnext * K[f]);
+          stv<CPW>(uf, g, a0 + bnext * K[f]);
         }
       } else {
         V an[NV];
@@ -682,12 +684,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
             else c0 = ldv<V>(cf, g * W);
             const V y = dt * K[f] - c0;
             const V t = a0 + y;
-            stv<CPS>(cf, g, (t - a0) - y);
+            stv<CPW>(cf, g, (t - a0) - y);
             an[f] = t;
           } else {
             an[f] = a0 + dt * K[f];
           }
-          stv<CPS>(af, g, an[f]);
+          stv<CPW>(af, g, an[f]);
         }
         // diagnostics: KE norm on the new state, minima on the stage-5 state
         V p0v;
