@@ -253,6 +253,12 @@ static void choose_tile(sparc_ctx* c) {
         break;
       }
     }
+    // quiet: the kernel runs fastest with about 512 blocks of full-depth
+    // chunks; a plane of more than 512 tiles of 32x8 takes 32x16 tiles
+    // instead (512x512x300 FP64, one stage: 32x8 2.61 ms at 150-plane
+    // chunks, 2.53 ms at 300; 32x16 2.22 ms at 300)
+    const long t32x8 = (long)((c->nx + 32 * c->vw - 1) / (32 * c->vw)) * ((c->ny + 7) / 8);
+    if (!c->mhd && tx == 32 && ty == 8 && t32x8 > 512 && tile_fits(c, 32, 16)) ty = 16;
   }
   set_geometry(c, tx, ty, q.tile[2]);
 }
@@ -266,10 +272,11 @@ static void set_geometry(sparc_ctx* c, int tx, int ty, int kc) {
   long tiles = (long)c->ntx * c->nty;
   if (kc <= 0) {
     // the quiet kernel re-reads 10 planes of its z queue per chunk, so it
-    // keeps chunks deep (~2048 blocks); the MHD xy kernel has no queue and
-    // runs fastest with many shallow chunks (4 planes, measured on 256^3:
-    // 16x16 FP64 1.85 ms at 8 planes, 1.77 ms at 4)
-    long want = c->zp ? 16384 : 2048;
+    // keeps chunks deep (~512 blocks: 512x512x300 FP32 32x8 lanes, one stage
+    // 1.41 ms at 2048 blocks, 1.30 at 1024, 1.23 at 512); the MHD xy kernel
+    // has no queue and runs fastest with many shallow chunks (4 planes,
+    // measured on 256^3: 16x16 FP64 1.85 ms at 8 planes, 1.77 ms at 4)
+    long want = c->zp ? 16384 : 512;
     kc = (int)std::max<long>(4, (c->nz * tiles + want - 1) / want);
     kc = std::min(kc, c->nz);
   }
