@@ -69,7 +69,10 @@ typedef struct sparc_cfg {
 } sparc_cfg;
 
 /* Backgrounds after INITIALISE_RHS (initialize.f90:883-1228), nondimensional.
- * The first group is nz-long when cfg.bg3d == 0, else nx*ny*nz. */
+ * The first group is nz-long when cfg.bg3d == 0, else nx*ny*nz.  When rhoinv
+ * is bitwise 1.0/rho0 at every point (initialize.f90:1012 makes it so), the
+ * FP64 MHD stage forms it by division instead of reading it from HBM; the
+ * environment variable SPARC_RHOINV=load keeps the read. */
 typedef struct sparc_bg {
   const double *rho0, *rhoinv, *c2rho0, *gradrho0_z, *gradp0_z, *spongexyz, *p0;
   const double* g; /* nz, PML-scaled gravity */

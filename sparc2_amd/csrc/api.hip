@@ -128,6 +128,8 @@ struct sparc_ctx {
   int tx = 64, ty = 8, kc = 8, ntx = 1, nty = 1, nchunk = 1;
   int zkc = 8, znchunk = 1;  // z-pass chunking
   bool bg_set = false, state_set = false;
+  // the uploaded rhoinv is bitwise 1/rho0 (initialize.f90:1012) at every point
+  bool rinv_rho0 = false;
   long steps_done = 0;
   double* op_buf = nullptr;  // 2*N doubles for the standalone operators
   bool timing = false;
@@ -485,6 +487,14 @@ int sparc_hip_set_background(sparc_ctx* c, const sparc_bg* bg) {
     if ((rc = upload(c, c->sy, bg->src_y, c->ny))) return rc;
     if ((rc = upload(c, c->sz, bg->src_z, c->nz))) return rc;
   }
+  // The FP64 MHD xy kernel recomputes rhoinv as 1/rho0 when the host's array
+  // holds exactly that (SPARC_RHOINV=load keeps reading it).
+  {
+    const char* ri = getenv("SPARC_RHOINV");
+    bool same = !c->f32 && c->mhd && !(ri && std::string(ri) == "load");
+    for (long i = 0; same && i < bgn; ++i) same = bg->rhoinv[i] == 1.0 / bg->rho0[i];
+    c->rinv_rho0 = same;
+  }
   c->dimc = bg->dimc;
   c->dimrho = bg->dimrho;
   c->bg_set = true;
@@ -611,6 +621,7 @@ static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
   A.znchunk = c->znchunk;
   A.periodic = c->cfg.periodic ? 1 : 0;
   A.pulse_dir = c->cfg.pulse_dir;
+  A.rinv_rho0 = c->rinv_rho0 ? 1 : 0;
   A.dt = (T)c->cfg.deltat;
   A.sxs = (T)c->cfg.stretchx;
   A.sys = (T)c->cfg.stretchy;

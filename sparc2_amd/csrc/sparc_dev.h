@@ -105,6 +105,10 @@ template <typename T> struct StageArgs {
   int ntx, nty, nblk_xy, nchunk;
   int zkc, znchunk;  // MHD z pass: its own z chunk (deeper: the queue warm-up re-reads 10 planes)
   int periodic, first, last, pulse_dir;
+  // rhoinv holds 1/rho0 at every point (checked bitwise on the host), so the
+  // FP64 MHD xy kernel divides (correctly rounded, the same bits) instead of
+  // reading the array
+  int rinv_rho0;
   T beta_next, dt, sxs, sys, src_amp;
   // when amp_tab is set the source amplitude is amp_tab[*tstep] (a device
   // table and step counter, so a captured step replays unchanged)

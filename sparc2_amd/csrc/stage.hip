@@ -380,9 +380,11 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       const T* const bgp[17] = {Ap->rho0, Ap->rhoinv, Ap->sponge, Ap->red, Ap->box, Ap->boy,
                                 Ap->boz, Ap->gr0x, Ap->gr0y, Ap->gr0z, Ap->c2rho0, Ap->gp0x,
                                 Ap->gp0y, Ap->gp0z, Ap->cbox, Ap->cboy, Ap->cboz};
+      // rhoinv (slot 1) reads nothing when the kernel forms it from rho0
+      const uint32_t bo_ri = (sizeof(T) == 8 && Ap->rinv_rho0) ? Buf<T>::kOut : bo;
 #pragma unroll
       for (int q2 = 0; q2 < 17; ++q2)
-        pm[MP_BG + q2] = (q2 >= 4 && q2 <= 6) ? B3b(bgp[q2], bo) : LDS1(bgp[q2], bo);
+        pm[MP_BG + q2] = (q2 >= 4 && q2 <= 6) ? B3b(bgp[q2], bo) : LDS1(bgp[q2], q2 == 1 ? bo_ri : bo);
       if (last) {
         pm[MP_P0] = LDS1(Ap->p0, bo);
         if constexpr (sizeof(T) == 4) {
@@ -595,7 +597,11 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         else if constexpr (MPX) return PMB(slot);  // rho0, rhoinv, sponge
         else return BG(p, g, k);
       };
-      const V rho0 = BGP(Ap->rho0, 0), rhoinv = BGP(Ap->rhoinv, 1), sp = BGP(Ap->sponge, 2);
+      const V rho0 = BGP(Ap->rho0, 0), sp = BGP(Ap->sponge, 2);
+      V rhoinv = BGP(Ap->rhoinv, 1);
+      if constexpr (MPX && sizeof(T) == 8) {
+        if (Ap->rinv_rho0) rhoinv = T(1) / rho0;  // IEEE division: the host's bits
+      }
       const V div = dvxdx + dvydy + dvzdz;
       V K[8];
       if constexpr (!MHD) {
