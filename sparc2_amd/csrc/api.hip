@@ -285,8 +285,9 @@ static void set_geometry(sparc_ctx* c, int tx, int ty, int kc) {
   c->kc = kc;
   c->nchunk = (c->nz + kc - 1) / kc;
   // z pass: about 512 blocks of 256 columns, at least 32 planes deep so the
-  // 10-plane queue warm-up stays a small overhead (FP32 measured slower with
-  // 1024 blocks: 0.26 -> 0.29 ms on 256^3, the warm-up outweighs occupancy)
+  // 10-plane queue warm-up stays a small overhead (256^3, two chunks of 128
+  // planes: FP64 0.471 ms, FP32 0.263 ms; one chunk of 256 planes, 256
+  // blocks: 0.492 / 0.391 ms; four chunks, 1024 blocks: 0.564 / 0.284 ms)
   const long ncb = ((long)c->nx * c->ny + 255) / 256;
   c->zkc = (int)std::min<long>(c->nz, std::max<long>(32, (c->nz * ncb + 511) / 512));
   c->znchunk = (c->nz + c->zkc - 1) / c->zkc;
