@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import Tuple
+from typing import Optional, Tuple
 
 RSUN = 69598946770.0          # all_modules.f90:56
 PI = 3.14159265358979         # all_modules.f90:56
@@ -26,7 +26,8 @@ class RunConfig:
     timestep: float = 1.0               # s
     solartime: float = 1.0              # h
     magnetic: int = 0                   # 0 quiet, 1 thermal, 2 MHD (MAGNETIC switch)
-    periodic: bool = True
+    # None: periodic in x and y unless USE_HPML is on
+    periodic: Optional[bool] = None
     use_pml: bool = True
     use_hpml: bool = False
     stabilise: bool = True
@@ -65,8 +66,18 @@ class RunConfig:
     precision: str = "fp64"             # "fp64" or "fp32" (FP32 fields + Kahan RK update)
 
     def __post_init__(self):
-        if self.use_hpml:
-            self.periodic = False       # initialize.f90:1610-1620
+        if self.periodic is None:
+            self.periodic = not self.use_hpml
+        elif self.periodic and self.use_hpml:
+            # A control file with PERIODIC = 1 before USE_HPML = 1 leaves both
+            # on in the reference (initialize.f90:1600-1624: the PERIODIC case
+            # only checks a USE_HPML already read).  That combination (wrapped
+            # x/y derivatives with x/y CPML faces) is not implemented here, so
+            # it is refused rather than changed into another run.
+            raise ValueError("PERIODIC and USE_HPML are both on (PERIODIC = 1 read before "
+                             "USE_HPML = 1): periodic x/y boundaries with horizontal PMLs are "
+                             "not supported; set PERIODIC = 0 or USE_HPML = 0")
+        self.periodic = bool(self.periodic)
         npml = list(self.npml)
         if not self.use_hpml:
             npml[0:4] = [0, 0, 0, 0]    # initialize.f90:1626-1648

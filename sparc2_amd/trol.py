@@ -306,6 +306,11 @@ def to_run_config(p: Dict, precision: str = "fp64") -> RunConfig:
         raise ParameterError("the PSR pulse driver (PULSE = 2) is not part of this package")
     if p["g_xy_filt"] == 0 or p["g_z_filt"] == 0:
         raise ParameterError("filter_mode_xy and filter_mode_z must be 1 or 2")
+    if p["PERIODIC"] and p["USE_HPML"]:
+        # PERIODIC = 1 read before USE_HPML = 1: the reference keeps both on
+        # (initialize.f90:1600-1624), a combination this package refuses
+        raise ParameterError("PERIODIC = 1 precedes USE_HPML = 1, which leaves both on: periodic "
+                             "x/y boundaries with horizontal PMLs are not supported")
     magnetic = 2 if p["MAGNETIC"] else (0 if p["QUIET"] else 1)
     npml = (p["nxpmlbot"], p["nxpmltop"], p["nypmlbot"], p["nypmltop"], p["nzpmlbot"], p["nzpmltop"])
     pmlc = tuple(tuple(p[k]) for k in ("xpmlb", "xpmlt", "ypmlb", "ypmlt", "zpmlb", "zpmlt"))

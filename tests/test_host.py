@@ -23,9 +23,12 @@ def test_derived_cadences_and_flags():
     assert c.maxtime == 2 * ((4 * 3600 * 4 // 3) // 2)
     c2 = RunConfig(nx=32, ny=32, nz=32, xlength=1e9, ylength=1e9, timestep=2.0, t_xy=1.0, t_z=0.5)
     assert (c2.xyfreq, c2.zfreq) == (1, 1)            # t < timestep is clamped
-    c3 = RunConfig(nx=32, ny=32, nz=32, xlength=1e9, ylength=1e9, use_hpml=True, periodic=True,
+    c3 = RunConfig(nx=32, ny=32, nz=32, xlength=1e9, ylength=1e9, use_hpml=True,
                    npml=(4, 4, 4, 4, 6, 6))
     assert c3.periodic is False                       # USE_HPML switches PERIODIC off
+    assert RunConfig(nx=32, ny=32, nz=32, xlength=1e9, ylength=1e9).periodic is True
+    with pytest.raises(ValueError, match="PERIODIC and USE_HPML"):
+        RunConfig(nx=32, ny=32, nz=32, xlength=1e9, ylength=1e9, use_hpml=True, periodic=True)
     c4 = RunConfig(nx=32, ny=32, nz=32, xlength=1e9, ylength=1e9, use_hpml=False,
                    npml=(4, 4, 4, 4, 6, 6))
     assert c4.npml == (0, 0, 0, 0, 6, 6)

@@ -88,6 +88,22 @@ def test_run_config_mapping():
         to_run_config(parse("TWOD = 1\n"))
 
 
+def test_periodic_before_hpml_is_refused():
+    """PERIODIC = 1 then USE_HPML = 1 leaves both on in the reference
+    (initialize.f90:1600-1624); the run config refuses it instead of running
+    a different case.  The other order turns periodicity off and is accepted."""
+    body = ("nx = 64\nny = 64\nnz = 64\nxlength = 50.0d8\nylength = 50.0d8\ntimestep = 1.0\n"
+            "MAGNETIC = 0\nUSE_PML = 1\nxpmlt = 10 9.2103 2 1 0.1\nxpmlb = 10 9.2103 2 1 0.1\n"
+            "ypmlt = 10 9.2103 2 1 0.1\nypmlb = 10 9.2103 2 1 0.1\nfilter_mode_xy = 1\n"
+            "filter_mode_z = 1\n")
+    both = parse("PERIODIC = 1\nUSE_HPML = 1\n" + body)
+    assert both["PERIODIC"] and both["USE_HPML"]
+    with pytest.raises(ParameterError, match="USE_HPML"):
+        to_run_config(both)
+    c = to_run_config(parse("USE_HPML = 1\nPERIODIC = 1\n" + body))
+    assert c.use_hpml and not c.periodic
+
+
 @pytest.mark.skipif(not os.path.exists(REF), reason="reference tree not present")
 def test_reference_parameter_file():
     p = read_parameters(REF)
