@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# SPARC_ORACLE_LIB names another build of the oracle: the mutation tests load
+# copies with planted mistakes through it.
+LIB_PATH = os.environ.get("SPARC_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 
 class OrcCfg(C.Structure):
@@ -39,6 +41,8 @@ class OrcCfg(C.Structure):
 
 
 def build(force: bool = False) -> str:
+    if os.environ.get("SPARC_ORACLE_LIB"):
+        return LIB_PATH
     src = os.path.join(HERE, "sparc_oracle.c")
     if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "liboracle.so"],
