@@ -95,8 +95,11 @@ def profiled_traffic(workload, precision, tile, lane_points=1):
         return None, None
     if list(summ.get("tile") or []) != list(tile[:2]) or summ.get("lane_points", 1) != lane_points:
         return None, None
-    k = summ["kernels"].get("k_stage")
-    return (k["traffic_bytes_per_launch"] if k else None), os.path.relpath(path, ROOT)
+    # one RK stage: the stage kernel plus, for MHD, the z pass before it
+    ks = [summ["kernels"].get(n) for n in ("k_stage", "k_zpass")]
+    if not ks[0]:
+        return None, None
+    return sum(int(k["traffic_bytes_per_launch"]) for k in ks if k), os.path.relpath(path, ROOT)
 
 
 def cpu_baseline(workload, precision, steps, threads):
@@ -177,10 +180,13 @@ def mode_batch(args):
         s.sync()
     K = args.steps
     t0 = time.perf_counter()
-    b.advance(K)
+    b.advance(K, check_guard=False)
     for s in sol:
         s.sync()
     el = time.perf_counter() - t0
+    failed = b.failures()
+    if failed:
+        raise RuntimeError(f"runs failed the reference's checks in the timed steps: {failed}")
     pts = sum(c.npoints for c, _ in pop)
     groups, blocks = b.info()
     n_mhd = sum(1 for c, _ in pop if c.mhd)
@@ -271,6 +277,62 @@ def mode_profile(args):
     s.close()
 
 
+def stream_ceiling(n=1 << 26, reps=20, reads=(1, 2, 4, 8)):
+    """Measured bandwidth of the library's own stream probe (nread arrays
+    summed into one, 16-byte non-temporal accesses) at several read-to-write
+    ratios: the ceiling the stage kernels' fractions are also read against."""
+    import ctypes as C
+    from sparc2_amd._lib import check, lib
+    out = {}
+    for r in reads:
+        g = C.c_double()
+        check(lib().sparc_hip_stream_bench(0, n, r, reps, C.byref(g)), "sparc_hip_stream_bench")
+        out[f"{r}:1"] = round(g.value, 1)
+    log(f"[stream] GB/s by reads:writes {out}")
+    return out
+
+
+def precision_form(marks=(500, 1000, 2000)):
+    """The precision form on quiet 512x512x300: FP32 fields with the
+    compensated RK update against plain FP64, the same driver loop on both;
+    relative L2 of every field at each mark, throughput of each over the
+    whole run (state resident; set-up and read-outs excluded)."""
+    from sparc2_amd import workloads
+    from sparc2_amd.solver import Solver
+    cfg64, bg = workloads.build("quiet_precision", precision="fp64")
+    t0 = time.perf_counter()
+    s64 = Solver(cfg64, bg)
+    cfg32 = cfg64.replace(precision="fp32")
+    s32 = Solver(cfg32, bg, setup=s64.st)
+    log(f"[precision] set-up {time.perf_counter() - t0:.1f} s")
+    names = ["rho", "vx", "vy", "vz", "p"]
+    l2, t = {}, {"fp64": 0.0, "fp32": 0.0}
+    n0 = 0
+    for m in marks:
+        for key, s in (("fp64", s64), ("fp32", s32)):
+            s.sync()
+            t1 = time.perf_counter()
+            s.advance(n0, m - 1)        # ends with the guard read (one sync)
+            t[key] += time.perf_counter() - t1
+        a64, a32 = s64.get_state(), s32.get_state()
+        l2[str(m)] = {f: float(np.linalg.norm(a32[q] - a64[q]) / max(np.linalg.norm(a64[q]), 1e-300))
+                      for q, f in enumerate(names)}
+        log(f"[precision] step {m}: {l2[str(m)]}")
+        n0 = m
+    n = marks[-1]
+    g64 = cfg64.npoints * n / t["fp64"] / 1e9
+    g32 = cfg64.npoints * n / t["fp32"] / 1e9
+    b64, b32 = alg_bytes_step(cfg64, 8) / cfg64.npoints, alg_bytes_step(cfg32, 4) / cfg64.npoints
+    s64.close()
+    s32.close()
+    return {"workload": "quiet_precision", "grid": [cfg64.nx, cfg64.ny, cfg64.nz], "steps": n,
+            "fp64_gpts": round(g64, 4), "fp32_gpts": round(g32, 4), "speedup": round(g32 / g64, 3),
+            "alg_bytes_per_point": {"fp64": round(b64, 1), "fp32": round(b32, 1)},
+            "hbm_frac": {"fp64": round(b64 * g64 / HBM_PEAK_GBS, 4),
+                         "fp32": round(b32 * g32 / HBM_PEAK_GBS, 4)},
+            "rel_l2": l2, "max_rel_l2": max(max(v.values()) for v in l2.values())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +343,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="leave out the FP32-vs-FP64 comparison on quiet 512x512x300")
     ap.add_argument("--tile", default="0,0,0")
     ap.add_argument("--mode", default="run", choices=["run", "precision", "batch", "serve", "profile"],
                     help="run: the headline single-run line (default); precision: FP32+Kahan "
@@ -320,17 +384,19 @@ def main():
     s.sync()
     log(f"[bench] warm-up {W} steps done")
     t0 = time.perf_counter()
-    s.advance(W, W + K - 1)
+    s.advance(W, W + K - 1, check_guard=False)
     s.sync()
     elapsed = time.perf_counter() - t0
     ms_step = elapsed / K * 1e3
     gpts = cfg.npoints * K / elapsed / 1e9
+    # the timed steps passed the reference's failure checks (a run stopped by
+    # them would skip its kernels and time nothing)
+    s.check_guard()
     d = s.diag()
     log(f"[bench] {K} steps in {elapsed:.3f} s: {ms_step:.3f} ms/step, {gpts:.3f} GPt/s, "
         f"diag {d}")
 
-    # dominant kernel: the middle RK stages, timed with HIP events on the
-    # context's stream
+    # stage kernels timed with HIP events on the context's stream
     P = args.profile_steps
     s.timing(True)
     s.advance(W + K, W + K + P - 1)
@@ -343,24 +409,24 @@ def main():
     n_mid, avg_mid = times["stage2_4"]
     bytes_stage = alg_bytes_stage(cfg, w)
     zpass = times["zpass"][0] > 0
-    # the dominant kernel: the stage kernel of RK stages 2-4 (for MHD the xy
-    # kernel, whose inputs include the z pass's six derivative arrays)
-    bytes_kernel = alg_bytes_xy(cfg, w) if zpass else bytes_stage
-    achieved = bytes_kernel / (avg_mid * 1e-3) / 1e9
-    log(f"[bench] stage kernel avg {avg_mid:.4f} ms over {n_mid} launches, "
-        f"{bytes_kernel / 1e9:.3f} GB algorithmic -> {achieved:.1f} GB/s; other kernels {times}")
-    per_kernel = {"k_stage": {"avg_ms": round(avg_mid, 5), "alg_bytes": int(bytes_kernel),
-                              "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+    bytes_xy = alg_bytes_xy(cfg, w) if zpass else bytes_stage
+    per_kernel = {"k_stage": {"avg_ms": round(avg_mid, 5), "alg_bytes": int(bytes_xy),
+                              "frac": round(bytes_xy / (avg_mid * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
     stage_ms = avg_mid
+    split_extra = 0
     if zpass:
         zb = alg_bytes_zpass(cfg, w)
         zms = times["zpass"][1]
         per_kernel["k_zpass"] = {"avg_ms": round(zms, 5), "alg_bytes": int(zb),
                                  "frac": round(zb / (zms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         stage_ms = avg_mid + zms
-    # one whole RK stage against the unsplit minimum of its traffic
-    stage_frac = bytes_stage / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
-    log(f"[bench] RK stage {stage_ms:.4f} ms, {bytes_stage / 1e9:.3f} GB minimum -> {stage_frac:.4f} of peak")
+        # the z derivatives handed from the z pass to the xy kernel (written,
+        # read back) and the z pass's own reads: bytes the split adds
+        split_extra = int(bytes_xy + zb - bytes_stage)
+    # graded figure: one middle RK stage against its unsplit algorithmic bytes
+    achieved = bytes_stage / (stage_ms * 1e-3) / 1e9
+    log(f"[bench] RK stage {stage_ms:.4f} ms, {bytes_stage / 1e9:.3f} GB algorithmic -> "
+        f"{achieved:.1f} GB/s = {achieved / HBM_PEAK_GBS:.4f} of peak; kernels {times}")
     step_bytes = alg_bytes_step(cfg, w)
     tile_used, nblk = s.tile()
     # the library gives FP32 lanes two x points on grids with even nx
@@ -368,6 +434,17 @@ def main():
                   and os.environ.get("SPARC_LANE_POINTS", "") != "1") else 1
     traffic, traffic_src = profiled_traffic(args.workload, args.precision, tile_used, lanes)
     s.close()
+
+    ceiling = stream_ceiling()
+    # the stage's read-to-write mix: MHD (u, a, 16 backgrounds) : un per
+    # field ~ 4:1, quiet ~ 2:1
+    ceil_key = "4:1" if cfg.mhd else "2:1"
+    prec = None
+    if not args.no_precision:
+        prec = precision_form()
+        for key in ("fp64", "fp32"):
+            prec["hbm_frac"][f"{key}_vs_stream_2:1"] = round(
+                prec["alg_bytes_per_point"][key] * prec[f"{key}_gpts"] / ceiling["2:1"], 4)
 
     cpu = None
     if not args.no_cpu:
@@ -402,8 +479,6 @@ def main():
             "step_hbm_frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "tile": list(tile_used) + [nblk],
             "kernel_avg_ms": {k: (round(v[1], 5) if v[1] else None) for k, v in times.items()},
-            "rk_stage": {"ms": round(stage_ms, 5), "alg_bytes_min": int(bytes_stage),
-                         "frac": round(stage_frac, 4)},
         },
         "roofline": {
             "bound": "hbm",
@@ -413,9 +488,17 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "k_stage (RK stages 2-4)" + (", MHD xy kernel after k_zpass" if zpass else ""),
+            "kernel": ("one middle RK stage = k_zpass + k_stage (MHD xy kernel); achieved = the "
+                       "stage's unsplit algorithmic bytes / (z pass + xy kernel time)"
+                       if zpass else "k_stage, RK stages 2-4"),
+            "alg_bytes": int(bytes_stage),
+            "stage_ms": round(stage_ms, 5),
+            "split_extra_alg_bytes": split_extra,
+            "frac_of_stream_ceiling": round(achieved / ceiling[ceil_key], 4),
+            "stream_ceiling_GBps": ceiling,
             "per_kernel": per_kernel,
         },
+        "precision_form": prec,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -118,7 +118,11 @@ int sparc_hip_set_forcing(sparc_ctx* ctx, const double* map, const double* sourc
 /* State in / out: a is nv*nx*ny*nz (field-major, the reference's `a`,
  * all_modules.f90:104); psi[f] is npsi*face_size for each active face (the
  * PML bases zpmlvars/xpmlvars/ypmlvars), NULL for inactive faces or to leave
- * them untouched (put) / skip them (get).  put_state clears the stage buffers. */
+ * them untouched (put) / skip them (get).  put_state clears the stage buffers.
+ * In FP32 a state value is held as a pair (v, c) meaning v - c (the Kahan
+ * compensation of the RK update): get_state returns v - c in FP64 and
+ * put_state splits x into v = fl32(x), c = fl32(v - x), so a state written
+ * from an FP32 run and read back continues as the uninterrupted run. */
 int sparc_hip_put_state(sparc_ctx* ctx, const double* a, const double* const psi[6]);
 int sparc_hip_get_state(sparc_ctx* ctx, double* a, double* const psi[6]);
 
@@ -138,6 +142,27 @@ int sparc_hip_advance(sparc_ctx* ctx, int n0, int n1, const double* src_amp);
  * (driver.f90:161), out[1] = min(rho+rho0), out[2] = min(p+p0) over the
  * stage-5 state (driver.f90:180).  Synchronises. */
 int sparc_hip_diag(sparc_ctx* ctx, double out[3]);
+
+/* The reference's failure checks of every step (driver.f90:175-196:
+ * norm_KE > 1e3, min(rho+rho0) <= 0, min(p+p0) <= 0), evaluated on the
+ * device by each step's diagnostic reduction, with no host synchronisation.
+ * A NaN norm or minimum also fails.  Modes: SPARC_GUARD_HALT (the default)
+ * records the first failure and stops the run there: every later STEP and
+ * FILTER_VARS of the context leaves the state alone, so it stays that of the
+ * failing step before its filter -- what the reference writes to
+ * full_XXXX.h5 before it stops (all_modules.f90:1090-1100).
+ * SPARC_GUARD_RECORD records the first failure and keeps stepping;
+ * SPARC_GUARD_OFF checks nothing.  set_guard also clears a recorded failure,
+ * and so does put_state. */
+#define SPARC_GUARD_OFF 0
+#define SPARC_GUARD_RECORD 1
+#define SPARC_GUARD_HALT 2
+int sparc_hip_set_guard(sparc_ctx* ctx, int mode);
+/* Synchronises; tripped = 1 after a failure, at_step = its step count (the
+ * n-th step the context completed, counting from 1), steps = steps the
+ * context has completed, diag = norm_KE (cgs), min(rho+rho0), min(p+p0) of
+ * the failing step.  Any output pointer may be NULL. */
+int sparc_hip_guard(sparc_ctx* ctx, int* tripped, long* at_step, long* steps, double diag[3]);
 
 /* Drop-in operators on one host FP64 array of nx*ny*nz (the context supplies
  * the metric).  bc: 1 (one-sided edges) or 5 (periodic) for x/y; 1 for z. */
@@ -194,16 +219,32 @@ int sparc_hip_get_slab(sparc_ctx* ctx, int field, int k0, int k1, double* out);
  * callers that run many forward simulations at once.
  * batch_advance: nsteps driver steps; time0[r] is run r's time index of the
  * first of them, amps[r*nsteps + n] its source amplitude at step n.  Filters
- * follow each run's own cadence.  Synchronises before returning. */
+ * follow each run's own cadence.  With flags = SPARC_DEFER_FILTER the
+ * filters of the last step are left out: the runs then hold the state the
+ * reference writes to its output files (after STEP, before FILTER_VARS,
+ * driver.f90:198-200), and sparc_hip_batch_filter(b, time) applies them,
+ * time[r] being run r's time index of that step.  Each run's failure guard
+ * works as for a single run.  Both synchronise before returning. */
+#define SPARC_DEFER_FILTER 1
 typedef struct sparc_batch sparc_batch;
 int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, sparc_batch** out);
-int sparc_hip_batch_advance(sparc_batch* b, int nsteps, const int* time0, const double* amps);
+int sparc_hip_batch_advance(sparc_batch* b, int nsteps, const int* time0, const double* amps,
+                            int flags);
+int sparc_hip_batch_filter(sparc_batch* b, const int* time);
 int sparc_hip_batch_info(sparc_batch* b, int* ngroups, long* nlaunch_blocks);
 int sparc_hip_batch_destroy(sparc_batch* b);
 /* Time per middle RK stage of the whole batch (reps launches of every
  * group), for tuning the tile shape of a group of runs; leaves every run's
  * state unchanged.  No counterpart in the reference. */
 int sparc_hip_batch_time_stage(sparc_batch* b, int reps, double* ms_out);
+
+/* Bandwidth probe (no counterpart in the reference): reps launches of a
+ * kernel reading nread arrays of n doubles and writing their sum to one
+ * more (16-byte non-temporal accesses), on device `device`; gbps = bytes
+ * moved per second / 1e9.  It allocates its own buffers.  The stage
+ * kernels' roofline fractions are read against it at their read-to-write
+ * ratios as well as against the data-sheet HBM peak. */
+int sparc_hip_stream_bench(int device, long n, int nread, int reps, double* gbps);
 
 const char* sparc_hip_strerror(int code);
 const char* sparc_hip_last_error(void);

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libsparc_hip.so")
 
 SPARC_OK = 0
 PREC_FP64, PREC_FP32 = 0, 1
+DEFER_FILTER = 1                         # SPARC_DEFER_FILTER
+GUARD_OFF, GUARD_RECORD, GUARD_HALT = 0, 1, 2
 
 DP = C.POINTER(C.c_double)
 
@@ -60,6 +62,7 @@ EXPORTS = [
     "sparc_hip_version", "sparc_hip_batch_create", "sparc_hip_batch_advance",
     "sparc_hip_batch_info", "sparc_hip_batch_destroy", "sparc_hip_alfven_search",
     "sparc_hip_get_slab", "sparc_hip_autotune", "sparc_hip_set_tile", "sparc_hip_batch_time_stage",
+    "sparc_hip_set_guard", "sparc_hip_guard", "sparc_hip_batch_filter", "sparc_hip_stream_bench",
 ]
 
 _lib = None
@@ -100,7 +103,11 @@ def lib() -> C.CDLL:
     L.sparc_hip_kernel_time.argtypes = [vp, i, C.POINTER(C.c_long), C.POINTER(C.c_double)]
     L.sparc_hip_tile.argtypes = [vp, C.c_int * 3, C.POINTER(C.c_long)]
     L.sparc_hip_batch_create.argtypes = [C.POINTER(vp), i, i, C.POINTER(vp)]
-    L.sparc_hip_batch_advance.argtypes = [vp, i, C.POINTER(C.c_int), DP]
+    L.sparc_hip_batch_advance.argtypes = [vp, i, C.POINTER(C.c_int), DP, i]
+    L.sparc_hip_batch_filter.argtypes = [vp, C.POINTER(C.c_int)]
+    L.sparc_hip_set_guard.argtypes = [vp, i]
+    L.sparc_hip_stream_bench.argtypes = [i, C.c_long, i, i, DP]
+    L.sparc_hip_guard.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_long), C.POINTER(C.c_long), DP]
     L.sparc_hip_batch_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_long)]
     L.sparc_hip_batch_destroy.argtypes = [vp]
     L.sparc_hip_alfven_search.argtypes = [vp, DP, DP, DP, C.c_double, C.c_double, C.c_double,

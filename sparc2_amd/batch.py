@@ -6,7 +6,10 @@
 Each run keeps its own time index; its source amplitudes (driver.f90:140-142)
 and filter cadence (derivatives.f90:669,690) follow that index.  The runs'
 states stay device-resident in their own contexts; ``Solver.get_state`` and
-``Solver.diag`` read them as usual.
+``Solver.diag`` read them as usual.  Each run's failure checks
+(driver.f90:175-196) run on the device every step; a run that fails stops
+at that step (its later work items are skipped) and ``advance`` reports it
+without disturbing the others.
 """
 from __future__ import annotations
 
@@ -15,8 +18,8 @@ from typing import Dict, List, Sequence
 
 import numpy as np
 
-from ._lib import SparcError, check, dptr, lib
-from .solver import Solver
+from ._lib import DEFER_FILTER, SparcError, check, dptr, lib
+from .solver import DivergenceError, Solver
 
 
 # Tile shapes the stage kernels are built for.
@@ -96,6 +99,7 @@ class Batch:
         self.tiles = tune_tiles(self.solvers) if autotune else None
         self.h = _create(self.solvers, use_graph)
         self.time = [0] * len(self.solvers)      # next time index of each run
+        self._deferred = None
 
     def time_stage(self, reps: int = 3) -> float:
         """Milliseconds per middle RK stage of the whole batch."""
@@ -107,16 +111,48 @@ class Batch:
         check(lib().sparc_hip_batch_info(self.h, C.byref(ng), C.byref(nb)), "sparc_hip_batch_info")
         return ng.value, nb.value
 
-    def advance(self, nsteps: int):
+    def advance(self, nsteps: int, defer_filter: bool = False, check_guard: bool = True):
+        """Every run advances nsteps driver steps.  defer_filter leaves out
+        the filters of the last step: the runs then hold the state the
+        reference writes out (after STEP, before FILTER_VARS, driver.f90:
+        198-200) until filter_deferred().  Returns {run index:
+        DivergenceError} of the runs whose failure checks have tripped."""
+        self.filter_deferred()
         n = len(self.solvers)
+        if nsteps <= 0:
+            return {}
         t0 = (C.c_int * n)(*self.time)
         amps = np.zeros((n, nsteps))
         for r, s in enumerate(self.solvers):
             if s.cfg.pulse:
                 amps[r] = [s.amplitude(self.time[r] + q) for q in range(nsteps)]
-        check(lib().sparc_hip_batch_advance(self.h, int(nsteps), t0, dptr(amps)),
+        check(lib().sparc_hip_batch_advance(self.h, int(nsteps), t0, dptr(amps),
+                                            DEFER_FILTER if defer_filter else 0),
               "sparc_hip_batch_advance")
+        for r, s in enumerate(self.solvers):
+            s._note(self.time[r], nsteps)
         self.time = [t + nsteps for t in self.time]
+        if defer_filter:
+            self._deferred = [t - 1 for t in self.time]
+        return self.failures() if check_guard else {}
+
+    def filter_deferred(self):
+        """Apply the filters advance(defer_filter=True) left out."""
+        if self._deferred is None:
+            return
+        t = (C.c_int * len(self._deferred))(*self._deferred)
+        self._deferred = None
+        check(lib().sparc_hip_batch_filter(self.h, t), "sparc_hip_batch_filter")
+
+    def failures(self):
+        """{run index: DivergenceError} of the runs whose failure checks
+        have tripped (synchronises each run's context)."""
+        out = {}
+        for r, s in enumerate(self.solvers):
+            failed, t, d, _ = s.guard()
+            if failed:
+                out[r] = DivergenceError(t, d)
+        return out
 
     def close(self):
         h = getattr(self, "h", None)

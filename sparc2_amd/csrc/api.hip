@@ -31,10 +31,11 @@ int zpass_batch_blocks(bool f32, int ncu);
 bool stage_tile_supported(int tx, int ty);
 template <typename T>
 hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
-                         int nfield, int periodic, T s, int bare, hipStream_t st);
+                         int nfield, int periodic, T s, int bare, hipStream_t st, const int* halt);
 hipError_t launch_deriv(int axis, const double* in, double* out, int nx, int ny, int nz, double sxy,
                         const double* stretch, int bc, hipStream_t st);
-hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, hipStream_t st);
+hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, const GuardArgs& g,
+                              hipStream_t st);
 hipError_t launch_zero(void* p, long n, bool f32, hipStream_t st);
 template <typename T>
 hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty,
@@ -43,7 +44,9 @@ hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int vw, int tx, int 
 int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, int ncu);
 hipError_t launch_diag_reduce_batch(const void* jobs, int njobs, hipStream_t st);
 hipError_t launch_tick(int* c, int reset, hipStream_t st);
-hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st);
+hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st, const int* halt);
+hipError_t launch_stream(const double* const* in, int nread, double* out, long n, int nblk,
+                         hipStream_t st);
 hipError_t launch_alfven_max(const double* red, const double* b2, const double* rho0, long n,
                              double dimb2, double T, double den0, double* part, int nblk,
                              hipStream_t st);
@@ -115,6 +118,7 @@ struct sparc_ctx {
   void* dzbuf = nullptr;
   void* bg[B_COUNT] = {};
   void *g = nullptr, *stretch = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
+  void* stretch_lo = nullptr;  // FP32: stretch(k) - fl32(stretch(k))
   double* stretch_d = nullptr;  // FP64 copy for the standalone operators
   FaceDev face[6];
   DampPlan* damp = nullptr;  // FFT damping (DAMPING = 1), null when off
@@ -124,6 +128,11 @@ struct sparc_ctx {
   double* partial = nullptr;
   long partial_cap = 0;  // blocks the partial buffer holds
   double* diag = nullptr;
+  // failure guard record (GuardWord layout) and the raw diagnostics of the
+  // failing step; host copy of the mode
+  int* guard = nullptr;
+  double* gdiag = nullptr;
+  int guard_mode = GUARD_HALT;
   double dimc = 1.0, dimrho = 1.0;
   int tx = 64, ty = 8, kc = 8, ntx = 1, nty = 1, nchunk = 1;
   int zkc = 8, znchunk = 1;  // z-pass chunking
@@ -190,12 +199,15 @@ static void free_all(sparc_ctx* c) {
   for (auto& p : c->bg) F(p);
   F(c->g);
   F(c->stretch);
+  F(c->stretch_lo);
   F(c->sx);
   F(c->sy);
   F(c->sz);
   F(c->stretch_d);
   F(c->partial);
   F(c->diag);
+  F(c->guard);
+  F(c->gdiag);
   F(c->op_buf);
   F(c->fmap);
   F(c->sdep);
@@ -411,6 +423,14 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
   if ((rc = dalloc((void**)&c->partial, sizeof(double) * 3 * nblk))) return bail(rc);
   c->partial_cap = nblk;
   if ((rc = dalloc((void**)&c->diag, sizeof(double) * 3))) return bail(rc);
+  if ((rc = dalloc((void**)&c->guard, sizeof(int) * G_WORDS))) return bail(rc);
+  if ((rc = dalloc((void**)&c->gdiag, sizeof(double) * 4))) return bail(rc);
+  {
+    int g0[G_WORDS] = {};
+    g0[G_MODE] = GUARD_HALT;
+    if (hipMemcpy(c->guard, g0, sizeof g0, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(SPARC_E_HIP, "guard upload failed"));
+  }
   for (int f = 0; f < 6; ++f) {
     FaceDev& F = c->face[f];
     F.active = (f < 4 ? q.use_hpml : q.use_pml) && q.npml[f] > 0;
@@ -438,6 +458,13 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
       if (launch_zero(c->comp[p], (long)c->nv * c->fs, true, c->st) != hipSuccess)
         return bail(fail(SPARC_E_HIP, "zero fill failed"));
   if ((rc = upload(c, c->stretch, q.stretch, q.nz))) return bail(rc);
+  if (c->f32) {
+    std::vector<float> lo(q.nz);
+    for (int k = 0; k < q.nz; ++k) lo[k] = (float)(q.stretch[k] - (double)(float)q.stretch[k]);
+    if ((rc = dalloc(&c->stretch_lo, sizeof(float) * q.nz))) return bail(rc);
+    if (hipMemcpy(c->stretch_lo, lo.data(), sizeof(float) * q.nz, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(SPARC_E_HIP, "stretch upload failed"));
+  }
   if (hipMemcpy(c->stretch_d, q.stretch, sizeof(double) * q.nz, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(SPARC_E_HIP, "stretch upload failed"));
   if (hipStreamSynchronize(c->st) != hipSuccess) return bail(fail(SPARC_E_HIP, "sync failed"));
@@ -539,10 +566,34 @@ int sparc_hip_set_forcing(sparc_ctx* c, const double* map, const double* source_
 int sparc_hip_put_state(sparc_ctx* c, const double* a, const double* const psi[6]) {
   if (!c) return fail(SPARC_E_INVALID, "null context");
   HIPCHK(hipSetDevice(c->dev));
+  if (c->f32)
+    for (int p = 0; p < 2; ++p)
+      if (launch_zero(c->comp[p], (long)c->nv * c->fs, true, c->st) != hipSuccess)
+        return fail(SPARC_E_HIP, "zero fill failed");
   if (a) {
-    for (int f = 0; f < c->nv; ++f) {
-      int rc = upload(c, (char*)c->pool[c->ra] + c->esz * (size_t)f * c->fs, a + (long)f * c->N, c->N);
-      if (rc) return rc;
+    if (!c->f32) {
+      for (int f = 0; f < c->nv; ++f) {
+        int rc = upload(c, (char*)c->pool[c->ra] + c->esz * (size_t)f * c->fs, a + (long)f * c->N, c->N);
+        if (rc) return rc;
+      }
+    } else {
+      // FP32: the value is held as hi - comp (the Kahan pair get_state
+      // reads back as a - comp), so an FP64 state -- one written from an
+      // FP32 run included -- comes back with its compensation:
+      // hi = fl32(x), comp = fl32(hi - x)
+      std::vector<float> hi(c->N), lo(c->N);
+      for (int f = 0; f < c->nv; ++f) {
+        const double* x = a + (long)f * c->N;
+        for (long q = 0; q < c->N; ++q) {
+          hi[q] = (float)x[q];
+          lo[q] = (float)((double)hi[q] - x[q]);
+        }
+        HIPCHK(hipMemcpyAsync((char*)c->pool[c->ra] + 4 * (size_t)f * c->fs, hi.data(), 4 * c->N,
+                              hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync((char*)c->comp[c->rc] + 4 * (size_t)f * c->fs, lo.data(), 4 * c->N,
+                              hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+      }
     }
   }
   for (int f = 0; f < 6; ++f) {
@@ -555,14 +606,13 @@ int sparc_hip_put_state(sparc_ctx* c, const double* a, const double* const psi[6
     if (launch_zero(F.pst, c->npsi * F.fsize, c->f32, c->st) != hipSuccess)
       return fail(SPARC_E_HIP, "zero fill failed");
   }
-  if (c->f32)
-    for (int p = 0; p < 2; ++p)
-      if (launch_zero(c->comp[p], (long)c->nv * c->fs, true, c->st) != hipSuccess)
-        return fail(SPARC_E_HIP, "zero fill failed");
   for (int p : {c->rk0, c->rk1})
     if (launch_zero(c->pool[p], (long)c->nv * c->fs, c->f32, c->st) != hipSuccess)
       return fail(SPARC_E_HIP, "zero fill failed");
   HIPCHK(hipStreamSynchronize(c->st));
+  // a new state starts with a clear guard (the step count runs on)
+  const int clear[3] = {0, 0, 0};
+  HIPCHK(hipMemcpy(c->guard, clear, sizeof clear, hipMemcpyHostToDevice));
   c->state_set = true;
   return SPARC_OK;
 }
@@ -626,6 +676,11 @@ static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
   A.dt = (T)c->cfg.deltat;
   A.sxs = (T)c->cfg.stretchx;
   A.sys = (T)c->cfg.stretchy;
+  // FP32: the rest of each FP64 scale (0 in FP64)
+  A.dt_lo = (T)(c->cfg.deltat - (double)A.dt);
+  A.sxs_lo = (T)(c->cfg.stretchx - (double)A.sxs);
+  A.sys_lo = (T)(c->cfg.stretchy - (double)A.sys);
+  A.stretch_lo = (const T*)c->stretch_lo;
   auto B = [&](int s) { return (const T*)c->bg[s]; };
   A.rho0 = B(B_RHO0);
   A.rhoinv = B(B_RHOINV);
@@ -666,6 +721,7 @@ static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
     D.pst = (T*)F.pst;
   }
   A.partial = c->partial;
+  A.halt = c->guard;
   for (int f = 0; f < 8; ++f) {
     A.comp[f] = c->f32 ? (T*)c->comp[c->rc] + (size_t)f * c->fs : nullptr;
     A.a[f] = (T*)c->pool[c->ra] + (size_t)f * c->fs;
@@ -685,10 +741,20 @@ static void set_stage(sparc_ctx* c, StageArgs<T>& A, int s) {
   A.first = s == 1;
   A.last = s == 5;
   A.beta_next = A.last ? (T)0 : (T)c->cfg.betas[s];
+  A.beta_lo = A.last ? (T)0 : (T)(c->cfg.betas[s] - (double)A.beta_next);
   for (int f = 0; f < 8; ++f) {
     A.u[f] = (const T*)c->pool[uin] + (size_t)f * c->fs;
     A.un[f] = A.last ? nullptr : (T*)c->pool[uout] + (size_t)f * c->fs;
   }
+}
+
+static GuardArgs guard_args(const sparc_ctx* c) {
+  GuardArgs g;
+  g.guard = c->guard;
+  g.gdiag = c->gdiag;
+  g.npts = (double)c->nx * (double)c->ny * (double)c->nz;
+  g.ke_scale = (c->dimc * c->dimc) * c->dimrho;
+  return g;
 }
 
 template <typename T>
@@ -716,7 +782,7 @@ static int do_step(sparc_ctx* c, double amp) {
     if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("stage launch: ") + hipGetErrorString(e));
   }
   const int nblk = c->ntx * c->nty * c->nchunk;
-  if (launch_diag_reduce(c->partial, nblk, c->diag, c->st) != hipSuccess)
+  if (launch_diag_reduce(c->partial, nblk, c->diag, guard_args(c), c->st) != hipSuccess)
     return fail(SPARC_E_HIP, "diag reduce launch failed");
   c->steps_done++;
   return SPARC_OK;
@@ -734,13 +800,13 @@ static int filter_pass(sparc_ctx* c, int axis, int g, T s) {
   TimeRec r;
   timed_begin(c, 3, &r);
   hipError_t e = launch_filter<T>(axis, g, (const T*)c->pool[c->ra], (T*)c->pool[c->rk0], c->nx,
-                                  c->ny, c->nz, c->fs, c->nv, c->cfg.periodic, s, 0, c->st);
+                                  c->ny, c->nz, c->fs, c->nv, c->cfg.periodic, s, 0, c->st, c->guard);
   timed_end(c, &r);
   if (e != hipSuccess) return fail(SPARC_E_HIP, "filter launch failed");
   std::swap(c->ra, c->rk0);
   if (c->f32) {  // the compensation term is filtered with the same linear operator
     e = launch_filter<T>(axis, g, (const T*)c->comp[c->rc], (T*)c->comp[1 - c->rc], c->nx, c->ny,
-                         c->nz, c->fs, c->nv, c->cfg.periodic, s, 0, c->st);
+                         c->nz, c->fs, c->nv, c->cfg.periodic, s, 0, c->st, c->guard);
     if (e != hipSuccess) return fail(SPARC_E_HIP, "filter launch failed");
     c->rc = 1 - c->rc;
   }
@@ -792,6 +858,45 @@ int sparc_hip_diag(sparc_ctx* c, double out[3]) {
   return SPARC_OK;
 }
 
+int sparc_hip_set_guard(sparc_ctx* c, int mode) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  if (mode != GUARD_OFF && mode != GUARD_RECORD && mode != GUARD_HALT)
+    return fail(SPARC_E_INVALID, "guard mode must be 0 (off), 1 (record) or 2 (halt)");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  int g[G_WORDS];
+  HIPCHK(hipMemcpy(g, c->guard, sizeof g, hipMemcpyDeviceToHost));
+  g[G_HALT] = g[G_TRIPPED] = g[G_AT] = 0;
+  g[G_MODE] = mode;
+  HIPCHK(hipMemcpy(c->guard, g, sizeof g, hipMemcpyHostToDevice));
+  c->guard_mode = mode;
+  return SPARC_OK;
+}
+
+int sparc_hip_guard(sparc_ctx* c, int* tripped, long* at_step, long* steps, double diag[3]) {
+  if (!c) return fail(SPARC_E_INVALID, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  int g[G_WORDS];
+  double d[4];
+  HIPCHK(hipMemcpy(g, c->guard, sizeof g, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(d, c->gdiag, sizeof d, hipMemcpyDeviceToHost));
+  if (tripped) *tripped = g[G_TRIPPED];
+  if (at_step) *at_step = g[G_TRIPPED] ? g[G_AT] : 0;
+  if (steps) *steps = g[G_COUNT];
+  if (diag) {
+    if (g[G_TRIPPED]) {
+      const double n = (double)c->nx * (double)c->ny * (double)c->nz;
+      diag[0] = std::pow(d[0] / n, 0.5) * (c->dimc * c->dimc) * c->dimrho;
+      diag[1] = d[1];
+      diag[2] = d[2];
+    } else {
+      diag[0] = diag[1] = diag[2] = 0.0;
+    }
+  }
+  return SPARC_OK;
+}
+
 static int op_common(sparc_ctx* c, const double* in, double* out) {
   if (!c || !in || !out) return fail(SPARC_E_INVALID, "null argument");
   HIPCHK(hipSetDevice(c->dev));
@@ -832,7 +937,7 @@ static int op_filt(sparc_ctx* c, int axis, const double* in, double* out) {
   // belongs to FILTER_VARS) and the result is filt itself
   const int g = axis == 2 ? c->cfg.g_z_filt : c->cfg.g_xy_filt;
   if (launch_filter<double>(axis, g, c->op_buf, c->op_buf + c->N, c->nx, c->ny, c->nz, c->N, 1,
-                            c->cfg.periodic, 0.0, 1, c->st) != hipSuccess)
+                            c->cfg.periodic, 0.0, 1, c->st, nullptr) != hipSuccess)
     return fail(SPARC_E_HIP, "filter launch failed");
   return op_finish(c, out);
 }
@@ -1075,7 +1180,7 @@ static int batch_filter(sparc_batch* b, sparc_ctx* c, int time) {
   auto pass = [&](int axis, int g, T sv, const void* in, void* out, bool comp) -> hipError_t {
     (void)comp;
     return launch_filter<T>(axis, g, (const T*)in, (T*)out, c->nx, c->ny, c->nz, c->fs, c->nv,
-                            c->cfg.periodic, sv, 0, b->st);
+                            c->cfg.periodic, sv, 0, b->st, c->guard);
   };
   void* a = c->pool[c->ra];
   void* t = c->pool[c->rk0];
@@ -1093,10 +1198,10 @@ static int batch_filter(sparc_batch* b, sparc_ctx* c, int time) {
   if (e == hipSuccess && time % c->cfg.zfreq == 0) {
     const T sv = (T)c->cfg.s_z;
     e = pass(2, c->cfg.g_z_filt, sv, a, t, false);
-    if (e == hipSuccess) e = launch_copy(t, a, n, c->f32, b->st);
+    if (e == hipSuccess) e = launch_copy(t, a, n, c->f32, b->st, c->guard);
     if (e == hipSuccess && c->f32) {
       e = pass(2, c->cfg.g_z_filt, sv, c->comp[c->rc], c->comp[1 - c->rc], true);
-      if (e == hipSuccess) e = launch_copy(c->comp[1 - c->rc], c->comp[c->rc], n, true, b->st);
+      if (e == hipSuccess) e = launch_copy(c->comp[1 - c->rc], c->comp[c->rc], n, true, b->st, c->guard);
     }
   }
   if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch filter: ") + hipGetErrorString(e));
@@ -1203,6 +1308,7 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
     jobs[r].out = c->diag;
     jobs[r].nblk = c->ntx * c->nty * c->nchunk;
     jobs[r].pad_ = 0;
+    jobs[r].g = guard_args(c);
   }
   if ((rc = dalloc(&b->jobs, sizeof(DiagJob) * nrun))) return bail(rc);
   if (hipMemcpy(b->jobs, jobs.data(), sizeof(DiagJob) * nrun, hipMemcpyHostToDevice) != hipSuccess)
@@ -1222,7 +1328,7 @@ int sparc_hip_batch_destroy(sparc_batch* b) {
 }
 
 template <typename T>
-static int batch_advance(sparc_batch* b, int nsteps, const int* time0, const double* amps) {
+static int batch_advance(sparc_batch* b, int nsteps, const int* time0, const double* amps, int flags) {
   const int nrun = (int)b->runs.size();
   int rc;
   if (nsteps > b->amp_cap) {  // grow the amplitude tables (device pointers change)
@@ -1259,6 +1365,10 @@ static int batch_advance(sparc_batch* b, int nsteps, const int* time0, const dou
     } else if ((rc = batch_enqueue_step<T>(b))) {
       return rc;
     }
+    // SPARC_DEFER_FILTER: the last step's filters wait for
+    // sparc_hip_batch_filter, so the caller can read the state the reference
+    // writes out, after STEP and before FILTER_VARS (driver.f90:198-200)
+    if (n == nsteps - 1 && (flags & SPARC_DEFER_FILTER)) break;
     for (int r = 0; r < nrun; ++r) {
       const int t = time0[r] + n;
       sparc_ctx* c = b->runs[r];
@@ -1271,11 +1381,33 @@ static int batch_advance(sparc_batch* b, int nsteps, const int* time0, const dou
   return SPARC_OK;
 }
 
-int sparc_hip_batch_advance(sparc_batch* b, int nsteps, const int* time0, const double* amps) {
+int sparc_hip_batch_advance(sparc_batch* b, int nsteps, const int* time0, const double* amps,
+                            int flags) {
   if (!b || nsteps < 0 || (nsteps > 0 && (!time0 || !amps))) return fail(SPARC_E_INVALID, "bad argument");
+  if (flags & ~SPARC_DEFER_FILTER) return fail(SPARC_E_INVALID, "unknown batch advance flag");
   if (nsteps == 0) return SPARC_OK;
   HIPCHK(hipSetDevice(b->dev));
-  return b->f32 ? batch_advance<float>(b, nsteps, time0, amps) : batch_advance<double>(b, nsteps, time0, amps);
+  return b->f32 ? batch_advance<float>(b, nsteps, time0, amps, flags)
+                : batch_advance<double>(b, nsteps, time0, amps, flags);
+}
+
+template <typename T>
+static int batch_filter_all(sparc_batch* b, const int* time) {
+  int rc;
+  for (size_t r = 0; r < b->runs.size(); ++r) {
+    sparc_ctx* c = b->runs[r];
+    const int t = time[r];
+    if (t != 0 && (t % c->cfg.xyfreq == 0 || t % c->cfg.zfreq == 0))
+      if ((rc = batch_filter<T>(b, c, t))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(b->st));
+  return SPARC_OK;
+}
+
+int sparc_hip_batch_filter(sparc_batch* b, const int* time) {
+  if (!b || !time) return fail(SPARC_E_INVALID, "null argument");
+  HIPCHK(hipSetDevice(b->dev));
+  return b->f32 ? batch_filter_all<float>(b, time) : batch_filter_all<double>(b, time);
 }
 
 // Average time of one middle RK stage of the whole batch (every group's z
@@ -1404,5 +1536,50 @@ int sparc_hip_get_slab(sparc_ctx* c, int field, int k0, int k1, double* out) {
     if (rc) return rc;
     for (long q = 0; q < n; ++q) out[q] -= cc[q];
   }
+  return SPARC_OK;
+}
+
+// ===========================================================================
+// Measured bandwidth ceiling (no counterpart in the reference)
+// ===========================================================================
+int sparc_hip_stream_bench(int device, long n, int nread, int reps, double* gbps) {
+  if (!gbps || n < 2 || n % 2 || nread < 1 || nread > 16 || reps < 1)
+    return fail(SPARC_E_INVALID, "bad argument");
+  HIPCHK(hipSetDevice(device));
+  std::vector<double*> buf(nread + 1, nullptr);
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = SPARC_OK;
+  auto cleanup = [&]() {
+    for (auto* p : buf)
+      if (p) (void)hipFree(p);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+  };
+  for (auto& p : buf)
+    if ((rc = dalloc((void**)&p, sizeof(double) * n))) {
+      cleanup();
+      return rc;
+    }
+  int ncu = 256;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  for (auto* p : buf)
+    if (e == hipSuccess) e = launch_zero(p, n, false, st);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  const int nblk = ncu * 8;
+  for (int q = 0; q <= reps && e == hipSuccess; ++q) {
+    if (q == 1) e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = launch_stream(buf.data(), nread, buf[nread], n, nblk, st);
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  cleanup();
+  if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("stream probe: ") + hipGetErrorString(e));
+  *gbps = (double)(nread + 1) * 8.0 * (double)n / (ms * 1e-3 / reps) / 1e9;
   return SPARC_OK;
 }

@@ -29,7 +29,11 @@ namespace sparc {
 template <typename T, int AXIS, int G>
 __global__ __launch_bounds__(256) void k_filter(const T* __restrict__ in, T* __restrict__ out,
                                                 int nx, int ny, int nz, long fs, int nseg,
-                                                int periodic, T s, int bare) {
+                                                int periodic, T s, int bare,
+                                                const int* __restrict__ halt) {
+  // a run stopped by its failure guard keeps its state: the pass copies
+  // (the host swaps the buffer roles after every pass)
+  const bool stop = halted(halt);
   const int nblk = gridDim.x, b = blockIdx.x;
   const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
   const int lg = xcd * base + (xcd < rem ? xcd : rem) + loc;
@@ -54,6 +58,10 @@ __global__ __launch_bounds__(256) void k_filter(const T* __restrict__ in, T* __r
   const int pos = AXIS == 0 ? i : AXIS == 1 ? j : k;
   const long stride = AXIS == 0 ? 1 : AXIS == 1 ? (long)nx : (long)nx * ny;
   const T* line = in + (long)f * fs + (idx - (long)pos * stride);
+  if (stop) {
+    out[(long)f * fs + idx] = line[(long)pos * stride];
+    return;
+  }
   const bool wrapped = (AXIS < 2) && periodic;
   // edge smoothing: derivatives.f90:671-675 (y), 680-684 (x), 690-693 (z)
   // (bare = the filtx/filty/filtz operators alone: no smoothing, result filt)
@@ -92,7 +100,8 @@ constexpr int ZPD = 4;
 template <typename T, int G>
 __global__ __launch_bounds__(256) void k_filter_z(const T* __restrict__ in, T* __restrict__ out,
                                                   int nx, int ny, int nz, long fs, int ncb, int kc,
-                                                  T s, int bare) {
+                                                  T s, int bare, const int* __restrict__ halt) {
+  const bool stop = halted(halt);  // guard stop: copy, as in k_filter
   const int nch = (nz + kc - 1) / kc;
   const int lg = blockIdx.x;
   const int cb = lg % ncb, rest = lg / ncb;
@@ -103,6 +112,10 @@ __global__ __launch_bounds__(256) void k_filter_z(const T* __restrict__ in, T* _
   const T* line = in + (long)f * fs + col;
   T* o = out + (long)f * fs + col;
   const int k0 = chunk * kc, k1 = min(k0 + kc, nz);
+  if (stop) {
+    for (int pos = k0; pos < k1; ++pos) o[(long)pos * nxy] = line[(long)pos * nxy];
+    return;
+  }
   auto ld = [&](int r) -> T { return (r >= 0 && r < nz) ? line[(long)r * nxy] : T(0); };
   constexpr int Q = 2 * G + 3;
   T q[Q], pf[ZPD];
@@ -139,7 +152,7 @@ __global__ __launch_bounds__(256) void k_filter_z(const T* __restrict__ in, T* _
 
 template <typename T>
 hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
-                         int nfield, int periodic, T s, int bare, hipStream_t st) {
+                         int nfield, int periodic, T s, int bare, hipStream_t st, const int* halt) {
   if (axis == 2) {
     // column blocks x z chunks x fields: >= ~2048 blocks, chunks >= 16 deep
     const long nxy = (long)nx * ny;
@@ -151,10 +164,10 @@ hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, i
     if (nb >= (1L << 31)) return hipErrorInvalidValue;
     if (g == 5)
       hipLaunchKernelGGL((k_filter_z<T, 5>), dim3((unsigned)nb), dim3(256), 0, st, in, out, nx, ny,
-                         nz, fs, ncb, kc, s, bare);
+                         nz, fs, ncb, kc, s, bare, halt);
     else
       hipLaunchKernelGGL((k_filter_z<T, 3>), dim3((unsigned)nb), dim3(256), 0, st, in, out, nx, ny,
-                         nz, fs, ncb, kc, s, bare);
+                         nz, fs, ncb, kc, s, bare, halt);
     return hipGetLastError();
   }
   // one block per x-row segment: 256 threads, or the row rounded up to
@@ -165,7 +178,7 @@ hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, i
   if (nb >= (1L << 31)) return hipErrorInvalidValue;
   dim3 grid((unsigned)nb);
 #define SPARC_F(AX, GG) \
-  hipLaunchKernelGGL((k_filter<T, AX, GG>), grid, dim3(nt), 0, st, in, out, nx, ny, nz, fs, nseg, periodic, s, bare)
+  hipLaunchKernelGGL((k_filter<T, AX, GG>), grid, dim3(nt), 0, st, in, out, nx, ny, nz, fs, nseg, periodic, s, bare, halt)
   if (g == 5) {
     if (axis == 0) SPARC_F(0, 5);
     else if (axis == 1) SPARC_F(1, 5);
@@ -179,9 +192,9 @@ hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, i
   return hipGetLastError();
 }
 template hipError_t launch_filter<double>(int, int, const double*, double*, int, int, int, long,
-                                          int, int, double, int, hipStream_t);
+                                          int, int, double, int, hipStream_t, const int*);
 template hipError_t launch_filter<float>(int, int, const float*, float*, int, int, int, long, int,
-                                         int, float, int, hipStream_t);
+                                         int, float, int, hipStream_t, const int*);
 
 // ---------------------------------------------------------------------------
 // Standalone derivatives (FP64, the reference's summation order)
@@ -241,9 +254,10 @@ hipError_t launch_deriv(int axis, const double* in, double* out, int nx, int ny,
 // ---------------------------------------------------------------------------
 // Diagnostic reduction of the per-block partials of the last RK stage.
 // One block, fixed order -> deterministic.  out = {sum m^2, min rho, min p}.
+// Thread 0 then counts the step and evaluates the failure guard.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_diag_reduce(const double* __restrict__ partial, int nblk,
-                                                     double* __restrict__ out) {
+__device__ void reduce_partials(const double* __restrict__ partial, int nblk, double* __restrict__ out,
+                                const GuardArgs& G) {
   __shared__ double s0[256], s1[256], s2[256];
   const int t = threadIdx.x;
   double a = 0.0, r = 1.0e300, p = 1.0e300;
@@ -268,38 +282,44 @@ __global__ __launch_bounds__(256) void k_diag_reduce(const double* __restrict__ 
     out[0] = s0[0];
     out[1] = s1[0];
     out[2] = s2[0];
+    if (G.guard) {
+      int* g = G.guard;
+      const int count = g[G_COUNT] + 1;
+      g[G_COUNT] = count;
+      if (g[G_MODE] != GUARD_OFF && g[G_TRIPPED] == 0) {
+        // driver.f90:175-183; a NaN norm or minimum also counts as failed
+        const double ke = sqrt(s0[0] / G.npts) * G.ke_scale;
+        if (!(ke <= 1.0e3) || !(s1[0] > 0.0) || !(s2[0] > 0.0)) {
+          G.gdiag[0] = s0[0];
+          G.gdiag[1] = s1[0];
+          G.gdiag[2] = s2[0];
+          g[G_AT] = count;
+          g[G_TRIPPED] = 1;
+          if (g[G_MODE] == GUARD_HALT) g[G_HALT] = 1;
+        }
+      }
+    }
   }
 }
 
-// Batched form: block r reduces run r's partials into its own out[3].
+__global__ __launch_bounds__(256) void k_diag_reduce(const double* __restrict__ partial, int nblk,
+                                                     double* __restrict__ out, GuardArgs G) {
+  if (halted(G.guard)) {  // the step is counted, nothing else
+    if (threadIdx.x == 0) G.guard[G_COUNT] += 1;
+    return;
+  }
+  reduce_partials(partial, nblk, out, G);
+}
 
+// Batched form: block r reduces run r's partials into its own out[3].
 __global__ __launch_bounds__(256) void k_diag_reduce_batch(const DiagJob* __restrict__ jobs) {
-  __shared__ double s0[256], s1[256], s2[256];
-  const DiagJob J = jobs[blockIdx.x];
-  const int t = threadIdx.x;
-  double a = 0.0, r = 1.0e300, p = 1.0e300;
-  for (int b = t; b < J.nblk; b += 256) {
-    a += J.partial[3 * (long)b];
-    r = fmin(r, J.partial[3 * (long)b + 1]);
-    p = fmin(p, J.partial[3 * (long)b + 2]);
+  const DiagJob& J = jobs[blockIdx.x];
+  if (halted(J.g.guard)) {  // the step is counted, nothing else
+    if (threadIdx.x == 0) J.g.guard[G_COUNT] += 1;
+    return;
   }
-  s0[t] = a;
-  s1[t] = r;
-  s2[t] = p;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (t < o) {
-      s0[t] += s0[t + o];
-      s1[t] = fmin(s1[t], s1[t + o]);
-      s2[t] = fmin(s2[t], s2[t + o]);
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    J.out[0] = s0[0];
-    J.out[1] = s1[0];
-    J.out[2] = s2[0];
-  }
+  const GuardArgs G = J.g;
+  reduce_partials(J.partial, J.nblk, J.out, G);
 }
 
 hipError_t launch_diag_reduce_batch(const void* jobs, int njobs, hipStream_t st) {
@@ -319,22 +339,25 @@ hipError_t launch_tick(int* c, int reset, hipStream_t st) {
 
 // Plain copy (the in-place filter of the batched form copies its result back).
 template <typename T>
-__global__ __launch_bounds__(256) void k_copy(const T* __restrict__ in, T* __restrict__ out, long n) {
+__global__ __launch_bounds__(256) void k_copy(const T* __restrict__ in, T* __restrict__ out, long n,
+                                              const int* __restrict__ halt) {
+  if (halted(halt)) return;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = in[i];
 }
 
-hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st) {
+hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st, const int* halt) {
   dim3 grid((unsigned)((n + 255) / 256));
   if (f32)
-    hipLaunchKernelGGL(k_copy<float>, grid, dim3(256), 0, st, (const float*)in, (float*)out, n);
+    hipLaunchKernelGGL(k_copy<float>, grid, dim3(256), 0, st, (const float*)in, (float*)out, n, halt);
   else
-    hipLaunchKernelGGL(k_copy<double>, grid, dim3(256), 0, st, (const double*)in, (double*)out, n);
+    hipLaunchKernelGGL(k_copy<double>, grid, dim3(256), 0, st, (const double*)in, (double*)out, n, halt);
   return hipGetLastError();
 }
 
-hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_diag_reduce, dim3(1), dim3(256), 0, st, partial, nblk, out);
+hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, const GuardArgs& g,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_reduce, dim3(1), dim3(256), 0, st, partial, nblk, out, g);
   return hipGetLastError();
 }
 
@@ -393,6 +416,57 @@ hipError_t launch_alfven_max(const double* red, const double* b2, const double* 
                              double dimb2, double T, double den0, double* part, int nblk,
                              hipStream_t st) {
   hipLaunchKernelGGL(k_alfven_max, dim3(nblk), dim3(256), 0, st, red, b2, rho0, n, dimb2, T, den0, part);
+  return hipGetLastError();
+}
+}  // namespace sparc
+
+namespace sparc {
+// ---------------------------------------------------------------------------
+// Stream probe: out[i] = sum_r in_r[i] over nread arrays, 16-byte loads and
+// stores, grid-stride with four independent elements per thread per trip.
+// Its achieved bandwidth at the read-to-write ratios of the stage kernels is
+// the measured ceiling their roofline fractions are read against (the HBM
+// peak of the data sheet is not reached by any access pattern).
+// ---------------------------------------------------------------------------
+typedef double sd2 __attribute__((ext_vector_type(2)));
+
+struct StreamArgs {
+  const sd2* in[16];
+  sd2* out;
+  long n2;  // 16-byte elements per array
+  int nread;
+};
+
+__global__ __launch_bounds__(256) void k_stream(StreamArgs A) {
+  const long stride = (long)gridDim.x * 256;
+  for (long i0 = (long)blockIdx.x * 256 + threadIdx.x; i0 < A.n2; i0 += 4 * stride) {
+    sd2 acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = (sd2)(0.0);
+    for (int r = 0; r < A.nread; ++r) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const long i = i0 + u * stride;
+        if (i < A.n2) acc[u] += __builtin_nontemporal_load(&A.in[r][i]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long i = i0 + u * stride;
+      if (i < A.n2) __builtin_nontemporal_store(acc[u], &A.out[i]);
+    }
+  }
+}
+
+hipError_t launch_stream(const double* const* in, int nread, double* out, long n, int nblk,
+                         hipStream_t st) {
+  if (nread < 1 || nread > 16 || n % 2) return hipErrorInvalidValue;
+  StreamArgs A{};
+  for (int r = 0; r < nread; ++r) A.in[r] = (const sd2*)in[r];
+  A.out = (sd2*)out;
+  A.n2 = n / 2;
+  A.nread = nread;
+  hipLaunchKernelGGL(k_stream, dim3(nblk), dim3(256), 0, st, A);
   return hipGetLastError();
 }
 }  // namespace sparc

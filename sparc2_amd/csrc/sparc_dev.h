@@ -59,8 +59,30 @@ template <> struct Coef<float> {
     constexpr double v[5] = SPARC_DPC;
     return (float)v[m];
   }
+  // the rest of the FP64 coefficient: c + cl carries it to ~1e-15
+  __device__ static float cl(int m) {
+    constexpr double v[5] = SPARC_DPC;
+    return (float)(v[m] - (double)(float)v[m]);
+  }
   __device__ static float b(int r, int m) { return (float)DPB_D[r][m]; }
 };
+
+// FP32 keeps each uniform scale of the scheme -- the derivative metric
+// (stretchx, stretchy, stretch(k)), dt and the stage weights, the centred
+// coefficients -- as a pair hi + lo of its FP64 value.  Rounded to one float
+// each would change the discrete wave speed and the time step by a fixed
+// ~1e-8 relative, a systematic phase error that grows linearly over
+// thousands of steps (about 1e-5 relative L2 after 2000); the lo parts add
+// one multiply-add each, in kernels bound by memory traffic.  x * (s + lo):
+template <typename T, typename X> __device__ __forceinline__ X scl(T s, T lo, X x) {
+  if constexpr (sizeof(T) == 4) {
+    const X h = s * x;
+    return h + lo * x;
+  } else {
+    (void)lo;
+    return s * x;
+  }
+}
 
 // Digital filters (initialize.f90:1932-1951): the 11-point filter is
 // 1 - sin^10(theta/2) (binomial(10,k)/1024), the 7-point one sin^6(theta/2).
@@ -110,6 +132,9 @@ template <typename T> struct StageArgs {
   // reading the array
   int rinv_rho0;
   T beta_next, dt, sxs, sys, src_amp;
+  // FP32: the remainders of the FP64 values (scl above); 0 in FP64
+  T beta_lo, dt_lo, sxs_lo, sys_lo;
+  const T* stretch_lo;  // nz remainders of stretch(k) (FP32 only)
   // when amp_tab is set the source amplitude is amp_tab[*tstep] (a device
   // table and step counter, so a captured step replays unchanged)
   const T* amp_tab;
@@ -131,6 +156,8 @@ template <typename T> struct StageArgs {
   const T *g, *stretch, *sx, *sy, *sz;
   Face<T> face[6];
   double* partial;  // last stage: 3 doubles per block (KE sum, min rho, min p)
+  // the run's guard record (Guard below); null: no guard
+  const int* halt;
 };
 
 // The stage kernels read their arguments through a pointer into the constant
@@ -144,12 +171,41 @@ template <typename T> using CPtr = const SPARC_AS4 T*;
 // being held in scalar registers across a loop.
 #define SPARC_REFRESH(p) asm volatile("" : "+s"(p))
 
+// Failure guard of one run (the reference's per-step checks, driver.f90:
+// 175-196): norm_KE > 1e3, min(rho+rho0) <= 0 or min(p+p0) <= 0.  The
+// diagnostic reduction of every step evaluates them on the device, so the
+// host needs no per-step synchronisation.  Layout of the int words:
+//   [0] halt: nonzero once a check has failed in halt mode; every later
+//       kernel of the run (stages, z pass, filters, reduction) then returns
+//       at once, so the state stays that of the failing step, before its
+//       filter -- the state the reference dumps before it stops
+//   [1] tripped (first failure recorded)   [2] step count at the failure
+//   [3] steps taken (counted by the reduction, halted ones included, so
+//       the count matches the host's)   [4] mode
+// and gdiag holds the raw reduction (sum of m^2, min rho, min p) of the
+// failing step.
+enum GuardWord { G_HALT = 0, G_TRIPPED = 1, G_AT = 2, G_COUNT = 3, G_MODE = 4, G_WORDS = 8 };
+enum GuardMode { GUARD_OFF = 0, GUARD_RECORD = 1, GUARD_HALT = 2 };
+
+__device__ __forceinline__ bool halted(const int* g) {
+  return g != nullptr && __atomic_load_n(g + G_HALT, __ATOMIC_RELAXED) != 0;
+}
+
+// Guard arguments of one run's reduction: the KE norm is
+// sqrt(sum / npts) * dimc^2 * dimrho in cgs (driver.f90:161).
+struct GuardArgs {
+  int* guard;
+  double* gdiag;
+  double npts, ke_scale;
+};
+
 // One run's diagnostic reduction in the batched form.
 struct DiagJob {
   const double* partial;
   double* out;
   int nblk;
   int pad_;
+  GuardArgs g;
 };
 
 // ---------------------------------------------------------------------------

@@ -128,7 +128,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
 
   // Array views, rebuilt from the argument pointer where they are used.
   // g: element of the lane's first point; bo: byte offset (Buf::kOut reads 0)
-  const T sxs = Ap->sxs, sys = Ap->sys;
+  const T sxs = Ap->sxs, sys = Ap->sys, sxl = Ap->sxs_lo, syl = Ap->sys_lo;
   constexpr uint32_t W = sizeof(T);
   auto UF = [&](int f, uint32_t g) -> V { return ldv<V>(Buf<T>(Ap->u[f], fs), g * W); };
   auto B3 = [&](const T* p, uint32_t g) -> V { return ldv<V>(Buf<T>(p, fs), g * W); };
@@ -222,11 +222,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     const V* row = &L[f][(ty + 5) * LX];
     V out;
     if constexpr (VW == 1) {
-      T acc = 0;
+      T acc = 0, acl = 0;
       const int c0 = tx + HX;
 #pragma unroll
-      for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (row[c0 + m] - row[c0 - m]);
-      out = sxs * acc;
+      for (int m = 1; m <= 5; ++m) {
+        const T d = row[c0 + m] - row[c0 - m];
+        acc += Coef<T>::c(m - 1) * d;
+        if constexpr (sizeof(T) == 4) acl += Coef<T>::cl(m - 1) * d;
+      }
+      if constexpr (sizeof(T) == 4) acc += acl;
+      out = scl(sxs, sxl, acc);
     } else {
       T e[14];  // points i-6 .. i+7
 #pragma unroll
@@ -235,13 +240,22 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         e[2 * q2] = w.x;
         e[2 * q2 + 1] = w.y;
       }
-      T a0 = 0, a1 = 0;
+      T a0 = 0, a1 = 0, l0 = 0, l1 = 0;
 #pragma unroll
       for (int m = 1; m <= 5; ++m) {
-        a0 += Coef<T>::c(m - 1) * (e[6 + m] - e[6 - m]);
-        a1 += Coef<T>::c(m - 1) * (e[7 + m] - e[7 - m]);
+        const T d0 = e[6 + m] - e[6 - m], d1 = e[7 + m] - e[7 - m];
+        a0 += Coef<T>::c(m - 1) * d0;
+        a1 += Coef<T>::c(m - 1) * d1;
+        if constexpr (sizeof(T) == 4) {
+          l0 += Coef<T>::cl(m - 1) * d0;
+          l1 += Coef<T>::cl(m - 1) * d1;
+        }
       }
-      out = V(sxs * a0, sxs * a1);
+      if constexpr (sizeof(T) == 4) {
+        a0 += l0;
+        a1 += l1;
+      }
+      out = V(scl(sxs, sxl, a0), scl(sxs, sxl, a1));
     }
     if (!periodic && (i < 5 || i + VW - 1 >= nx - 5)) {
       constexpr int XO = VW * HX, RW = VW * LX;
@@ -253,13 +267,13 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         if (p < 5) {
 #pragma unroll
           for (int m = 0; m < 11; ++m) acc += rp[min(m - i0 + XO, RW - 1)] * Coef<T>::b(p, m);
-          set_lane_pt(out, c, sxs * acc);
+          set_lane_pt(out, c, scl(sxs, sxl, acc));
         } else if (p >= nx - 5) {
           const int r = nx - 1 - p;
 #pragma unroll
           for (int m = 0; m < 11; ++m)
             acc += rp[min(max(nx - 1 - m - i0 + XO, 0), RW - 1)] * Coef<T>::b(r, m);
-          set_lane_pt(out, c, -sxs * acc);
+          set_lane_pt(out, c, -scl(sxs, sxl, acc));
         }
       }
     }
@@ -271,18 +285,24 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     if (!periodic && j < 5) {
 #pragma unroll
       for (int m = 0; m < 11; ++m) acc += col[min(max(m - j0 + 5, 0), LY - 1) * LX] * Coef<T>::b(j, m);
-      return sys * acc;
+      return scl(sys, syl, acc);
     }
     if (!periodic && j >= ny - 5) {
       const int r = ny - 1 - j;
 #pragma unroll
       for (int m = 0; m < 11; ++m) acc += col[min(max(ny - 1 - m - j0 + 5, 0), LY - 1) * LX] * Coef<T>::b(r, m);
-      return -sys * acc;
+      return -scl(sys, syl, acc);
     }
     const int r0 = ty + 5;
+    V acl = 0;
 #pragma unroll
-    for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (col[(r0 + m) * LX] - col[(r0 - m) * LX]);
-    return sys * acc;
+    for (int m = 1; m <= 5; ++m) {
+      const V d = col[(r0 + m) * LX] - col[(r0 - m) * LX];
+      acc += Coef<T>::c(m - 1) * d;
+      if constexpr (sizeof(T) == 4) acl += Coef<T>::cl(m - 1) * d;
+    }
+    if constexpr (sizeof(T) == 4) acc += acl;
+    return scl(sys, syl, acc);
   };
 
   // z register queue: slot m holds plane k-5+m of each z-differentiated
@@ -427,13 +447,20 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       fetch(k + 6, nq);
       if (active) {
         const T st = ((CPtr<T>)Ap->stretch)[k];
+        T stl = 0;
+        if constexpr (sizeof(T) == 4) stl = ((CPtr<T>)Ap->stretch_lo)[k];
         if (k < nz - 5) {
 #pragma unroll
           for (int f = 0; f < NQ; ++f) {
-            V acc = 0;
+            V acc = 0, acl = 0;
 #pragma unroll
-            for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (q[f][5 + m] - q[f][5 - m]);
-            dz[f] = st * acc;
+            for (int m = 1; m <= 5; ++m) {
+              const V d = q[f][5 + m] - q[f][5 - m];
+              acc += Coef<T>::c(m - 1) * d;
+              if constexpr (sizeof(T) == 4) acl += Coef<T>::cl(m - 1) * d;
+            }
+            if constexpr (sizeof(T) == 4) acc += acl;
+            dz[f] = scl(st, stl, acc);
           }
         } else {
           // top block: mirrored one-sided rows over planes nz-1 .. nz-11
@@ -454,7 +481,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
             }
           }
 #pragma unroll
-          for (int f = 0; f < NQ; ++f) dz[f] = -st * acc[f];
+          for (int f = 0; f < NQ; ++f) dz[f] = -scl(st, stl, acc[f]);
         }
       }
     }
@@ -673,7 +700,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         for (int f = 0; f < NV; ++f) {
           const Buf<T> uf(Ap->un[f], fs);
           const V a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : ldv<V>(Buf<T>(Ap->a[f], fs), g * W));
-          stv<CPW>(uf, g, a0 + bnext * K[f]);
+          stv<CPW>(uf, g, a0 + scl(bnext, Ap->beta_lo, K[f]));
         }
       } else {
         V an[NV];
@@ -688,7 +715,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
             if constexpr (NPC > 0) c0 = pc[IC + f];
             else if constexpr (MPX) c0 = pm[MP_C + f];
             else c0 = ldv<V>(cf, g * W);
-            const V y = dt * K[f] - c0;
+            const V y = scl(dt, Ap->dt_lo, K[f]) - c0;
             const V t = a0 + y;
             stv<CPW>(cf, g, (t - a0) - y);
             an[f] = t;
@@ -829,14 +856,21 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
     for (int f = 0; f < 6; ++f) q[f][10] = nq[f];
     fetch(k + 6, nq);
     const T st = ((CPtr<T>)Ap->stretch)[k];
+    T stl = 0;
+    if constexpr (sizeof(T) == 4) stl = ((CPtr<T>)Ap->stretch_lo)[k];
     T d[6];
     if (k < nz - 5) {
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
-        T acc = 0;
+        T acc = 0, acl = 0;
 #pragma unroll
-        for (int m = 1; m <= 5; ++m) acc += Coef<T>::c(m - 1) * (q[f][5 + m] - q[f][5 - m]);
-        d[f] = st * acc;
+        for (int m = 1; m <= 5; ++m) {
+          const T dd = q[f][5 + m] - q[f][5 - m];
+          acc += Coef<T>::c(m - 1) * dd;
+          if constexpr (sizeof(T) == 4) acl += Coef<T>::cl(m - 1) * dd;
+        }
+        if constexpr (sizeof(T) == 4) acc += acl;
+        d[f] = scl(st, stl, acc);
       }
     } else {
       // top block: mirrored one-sided rows over planes nz-1 .. nz-11
@@ -850,7 +884,7 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
         for (int f = 0; f < 6; ++f) acc[f] += v[f] * c;
       }
 #pragma unroll
-      for (int f = 0; f < 6; ++f) d[f] = -st * acc[f];
+      for (int f = 0; f < 6; ++f) d[f] = -scl(st, stl, acc[f]);
     }
     if (active) {
       const uint32_t g = ij + (uint32_t)k * nxy;
@@ -878,7 +912,9 @@ __global__ __launch_bounds__(ZP_NT, 2) void k_zpass(const StageArgs<T> A) {
   const int nblk = gridDim.x, b = blockIdx.x;
   const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
   const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
-  zpass_item<T>((CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr(), logical);
+  const CArgs<T> Ap = (CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr();
+  if (halted(Ap->halt)) return;  // the run's failure guard stopped it
+  zpass_item<T>(Ap, logical);
 }
 
 // Work distribution of the persistent batch kernels: the item list is cut
@@ -919,7 +955,8 @@ __global__ __launch_bounds__(ZP_NT, 2) void k_zpass_batch(const StageArgs<T>* __
   int r = 0;
   for (int item = next_item(work, lo, hi, &slot); item >= 0; item = next_item(work, lo, hi, &slot)) {
     while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
-    zpass_item<T>((CArgs<T>)(runs + r), item - item_start[r]);
+    const CArgs<T> Ap = (CArgs<T>)(runs + r);
+    if (!halted(Ap->halt)) zpass_item<T>(Ap, item - item_start[r]);
   }
   leave(work);
 }
@@ -933,7 +970,9 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP, VW>())) void
   const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
   const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
   // A sits at offset 0 of the kernel-argument segment
-  stage_item<T, MHD, BG3D, TX, TY, ZP, VW>((CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr(), logical);
+  const CArgs<T> Ap = (CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr();
+  if (halted(Ap->halt)) return;  // the run's failure guard stopped it
+  stage_item<T, MHD, BG3D, TX, TY, ZP, VW>(Ap, logical);
 }
 
 // Many runs in one persistent launch: the work items of all runs form one
@@ -952,8 +991,10 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP, VW>())) void
   int r = 0;
   for (int item = next_item(work, lo, hi, &slot); item >= 0; item = next_item(work, lo, hi, &slot)) {
     while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
-    // no kernel writes the argument array while the batch runs
-    stage_item<T, MHD, BG3D, TX, TY, ZP, VW>((CArgs<T>)(runs + r), item - item_start[r]);
+    // no kernel writes the argument array while the batch runs; an item of
+    // a run its failure guard stopped is skipped (uniform over the block)
+    const CArgs<T> Ap = (CArgs<T>)(runs + r);
+    if (!halted(Ap->halt)) stage_item<T, MHD, BG3D, TX, TY, ZP, VW>(Ap, item - item_start[r]);
   }
   leave(work);
 }

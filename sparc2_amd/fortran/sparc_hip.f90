@@ -56,7 +56,9 @@ module sparc_hip
        sparc_hip_alfven_search, sparc_hip_get_slab, sparc_hip_batch_create, &
        sparc_hip_batch_advance, sparc_hip_batch_info, sparc_hip_batch_destroy, &
        sparc_hip_autotune, sparc_hip_set_tile, sparc_hip_batch_time_stage, sparc_hip_set_damping, &
-       sparc_hip_set_forcing, sparc_error_message
+       sparc_hip_set_forcing, sparc_hip_set_guard, sparc_hip_guard, sparc_hip_batch_filter, &
+       sparc_hip_stream_bench, &
+       sparc_error_message
 
   interface
     integer(c_int) function sparc_hip_create(cfg, ctx) bind(C, name="sparc_hip_create")
@@ -234,14 +236,50 @@ module sparc_hip
       type(c_ptr), intent(out) :: batch
     end function
 
-    ! amps(nsteps, nrun): amps(n, r) is run r's source amplitude at step n
-    integer(c_int) function sparc_hip_batch_advance(batch, nsteps, time0, amps) &
+    ! amps(nsteps, nrun): amps(n, r) is run r's source amplitude at step n;
+    ! flags = 1 (SPARC_DEFER_FILTER) leaves the last step's filters to
+    ! sparc_hip_batch_filter, so outputs read the pre-filter state
+    integer(c_int) function sparc_hip_batch_advance(batch, nsteps, time0, amps, flags) &
          bind(C, name="sparc_hip_batch_advance")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: batch
       integer(c_int), value :: nsteps
       integer(c_int), intent(in) :: time0(*)
       real(c_double), intent(in) :: amps(*)
+      integer(c_int), value :: flags
+    end function
+
+    integer(c_int) function sparc_hip_batch_filter(batch, time) bind(C, name="sparc_hip_batch_filter")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), intent(in) :: time(*)
+    end function
+
+    ! bandwidth probe: nread arrays of n doubles summed into one more
+    integer(c_int) function sparc_hip_stream_bench(device, n, nread, reps, gbps) &
+         bind(C, name="sparc_hip_stream_bench")
+      import :: c_int, c_long, c_double
+      integer(c_int), value :: device
+      integer(c_long), value :: n
+      integer(c_int), value :: nread, reps
+      real(c_double), intent(out) :: gbps
+    end function
+
+    ! failure checks of driver.f90:175-196 on the device: mode 0 off,
+    ! 1 record, 2 halt at the failing step (default)
+    integer(c_int) function sparc_hip_set_guard(ctx, mode) bind(C, name="sparc_hip_set_guard")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+    end function
+
+    integer(c_int) function sparc_hip_guard(ctx, tripped, at_step, steps, diag) &
+         bind(C, name="sparc_hip_guard")
+      import :: c_int, c_ptr, c_long, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: tripped
+      integer(c_long), intent(out) :: at_step, steps
+      real(c_double), intent(out) :: diag(3)
     end function
 
     integer(c_int) function sparc_hip_batch_info(batch, ngroups, nlaunch_blocks) &

@@ -54,6 +54,9 @@ class Result:
     final: Optional[np.ndarray] = None
     points: int = 0
     steps: int = 0
+    # set when the run failed the reference's checks (driver.f90:175-196);
+    # it is retired at the failing step
+    error: Optional[Exception] = None
 
     @property
     def latency(self):
@@ -207,6 +210,7 @@ class Server:
                 key = tuple(sorted(active))
                 if key != members:
                     if batch is not None:
+                        batch.filter_deferred()
                         batch.close()
                     sol = [active[r]["solver"] for r in key]
                     batch = Batch(sol, use_graph=self.use_graph)
@@ -220,13 +224,22 @@ class Server:
                     to_slice = a["every"] - (a["done"] % a["every"])
                     chunk = min(chunk, left, to_slice)
                 tb = self.clock()
-                batch.advance(chunk)
+                # the slices are the reference's chunk output: the state after
+                # STEP and before that step's FILTER_VARS (driver.f90:198-200)
+                failed = batch.advance(chunk, defer_filter=True)
                 busy += self.clock() - tb
-                for r in key:
+                for q, r in enumerate(key):
                     a = active[r]
                     a["done"] += chunk
                     res = results[r]
                     res.steps = a["done"]
+                    if q in failed:
+                        # stopped at the failing step; retired below, once the
+                        # batch has no more work queued for it
+                        res.error = failed[q]
+                        res.steps = failed[q].time + 1
+                        say(f"[serve] t={now():7.2f}s fail  {r}: {failed[q]}")
+                        continue
                     if a["done"] % a["every"] == 0:
                         res.slices.append((a["done"], a["solver"].get_slab(VZ, a["k0"], a["k0"])[0]))
                         if self.chunk_dir:
@@ -234,8 +247,14 @@ class Server:
                             name = h5io.file_name(f"req{r}_", "chunk", len(res.slices))
                             h5io.write_chunk(os.path.join(self.chunk_dir, name), a["solver"], a["done"],
                                              zr=(a["k0"], a["k0"]))
-                    if a["done"] >= a["req"].nsteps:
-                        if self.keep_final:
+                batch.filter_deferred()
+                for r in key:
+                    a = active.get(r)
+                    if a is None:
+                        continue
+                    res = results[r]
+                    if res.error is not None or a["done"] >= a["req"].nsteps:
+                        if self.keep_final and res.error is None:
                             res.final = a["solver"].get_state()
                         res.finished = now()
                         a["solver"].close()

@@ -9,6 +9,12 @@ Method names mirror the reference routines they replace:
     Solver.norm_ke / diag     norm() + the negativity guards (driver.f90:161-183)
     Solver.run(n0, n1)        the driver loop body (driver.f90:123-200)
 
+The reference's failure checks (driver.f90:175-196) run on the device in
+every step's diagnostic reduction; ``advance`` reads the result once per
+call (one synchronisation) and raises DivergenceError naming the failing
+step.  In the default halt mode the device stops the run at that step, so
+the state it holds is the one the reference dumps to full_XXXX.h5.
+
 State arrays are numpy (nv, nz, ny, nx) FP64 == the reference's Fortran
 ``a(nx, ny, nz, nv)``.  PML bases are per face, (npsi, *face_shape).
 """
@@ -33,7 +39,18 @@ _TILE_CACHE: Dict[tuple, tuple] = {}
 
 
 class DivergenceError(RuntimeError):
-    """Raised by ``run`` when the reference's failure guards trip (driver.f90:175-183)."""
+    """The reference's failure checks tripped (driver.f90:175-196): norm_KE
+    above 1e3, or a non-positive rho+rho0 or p+p0, at time index ``time``;
+    ``diag`` = (norm_KE, min rho+rho0, min p+p0) of that step, ``dump`` the
+    full_XXXX.h5 written, if any."""
+
+    def __init__(self, time: int, diag, dump: Optional[str] = None):
+        self.time, self.diag, self.dump = time, tuple(float(v) for v in diag), dump
+        msg = (f"step {time}: norm_KE={self.diag[0]:.5e} min(rho+rho0)={self.diag[1]:.5e} "
+               f"min(p+p0)={self.diag[2]:.5e}")
+        if dump:
+            msg += f"; state written to {dump}"
+        super().__init__(msg)
 
 
 class Solver:
@@ -83,6 +100,11 @@ class Solver:
         # another one); every step uploads its forcing map first
         self.series: Optional[ForcingSeries] = None
         self._sdep_sent = False
+        # steps issued, and which time index each one had: (first count,
+        # first time, n) per contiguous stretch -- maps the guard's step
+        # count back to a time index
+        self._count = 0
+        self._segs = []
         if not cfg.pulse and cfg.forcingfunc:
             self.use_forcing(ForcingSeries.from_file(st))
 
@@ -175,44 +197,118 @@ class Solver:
         check(lib().sparc_hip_set_forcing(self.h, dptr(fmap), sdep), "sparc_hip_set_forcing")
         self._sdep_sent = True
 
+    def _note(self, time0: int, n: int):
+        if n <= 0:
+            return
+        if self._segs:
+            c0, t0, m = self._segs[-1]
+            if c0 + m == self._count + 1 and t0 + m == time0:
+                self._segs[-1] = (c0, t0, m + n)
+                self._count += n
+                return
+        self._segs.append((self._count + 1, int(time0), n))
+        self._count += n
+
+    def time_of_step(self, count: int) -> int:
+        """Time index of the count-th step this context took (from 1)."""
+        for c0, t0, m in reversed(self._segs):
+            if c0 <= count < c0 + m:
+                return t0 + count - c0
+        raise KeyError(count)
+
     def step(self, time: int):
+        """STEP at time index ``time`` (asynchronous; no guard check)."""
         if self.series is not None:
             self._push_forcing(time)
+        self._note(time, 1)
         check(lib().sparc_hip_step(self.h, self.amplitude(time)), "sparc_hip_step")
 
     def filter_vars(self, time: int):
         check(lib().sparc_hip_filter(self.h, int(time)), "sparc_hip_filter")
 
-    def advance(self, n0: int, n1: int):
+    def advance(self, n0: int, n1: int, check_guard: bool = True, filter_last: bool = True):
+        """The driver loop body for nn = n0..n1 (source, STEP, FILTER_VARS
+        when nn != 0).  filter_last=False leaves out the filter of step n1
+        (the caller writes the reference's outputs, then calls
+        filter_vars(n1)).  With check_guard the failure checks of every step
+        are read at the end (one synchronisation) and a failure raises
+        DivergenceError."""
+        if n1 < n0:
+            return
+        last = n1 if filter_last else n1 - 1
         if self.series is not None:
             # the forcing map changes every step: the driver loop runs here
             for t in range(n0, n1 + 1):
                 self.step(t)
-                if t != 0:
+                if t != 0 and t <= last:
                     self.filter_vars(t)
+        else:
+            if last >= n0:
+                amps = np.array([self.amplitude(t) for t in range(n0, last + 1)], dtype=np.float64)
+                self._note(n0, last - n0 + 1)
+                check(lib().sparc_hip_advance(self.h, int(n0), int(last), dptr(amps)),
+                      "sparc_hip_advance")
+            if last < n1:
+                self.step(n1)
+        if check_guard:
+            self.check_guard()
+
+    # ----------------------------------------------------------- guards
+    def set_guard(self, mode: int = _lib.GUARD_HALT):
+        """SPARC_GUARD_HALT (stop at the failing step, the default),
+        _RECORD (note the first failure, keep stepping) or _OFF; clears a
+        recorded failure."""
+        check(lib().sparc_hip_set_guard(self.h, int(mode)), "sparc_hip_set_guard")
+
+    def guard(self):
+        """(failed, time index of the failing step or None, its diag, steps
+        completed on the device).  Synchronises."""
+        tripped = C.c_int()
+        at = C.c_long()
+        done = C.c_long()
+        d = np.zeros(3)
+        check(lib().sparc_hip_guard(self.h, C.byref(tripped), C.byref(at), C.byref(done), dptr(d)),
+              "sparc_hip_guard")
+        if not tripped.value:
+            return False, None, None, done.value
+        return True, self.time_of_step(at.value), d, done.value
+
+    def check_guard(self, dump_dir: Optional[str] = None, simname: str = ""):
+        """Raise DivergenceError if a step failed the reference's checks;
+        with dump_dir the state the run stopped at is first written there as
+        <simname>full_XXXX.h5 (WRITE_OUT_FULL_STATE with fail set,
+        all_modules.f90:1090-1100)."""
+        failed, t, d, _ = self.guard()
+        if not failed:
             return
-        amps = np.array([self.amplitude(t) for t in range(n0, n1 + 1)], dtype=np.float64)
-        check(lib().sparc_hip_advance(self.h, int(n0), int(n1), dptr(amps)), "sparc_hip_advance")
+        path = None
+        if dump_dir is not None:
+            from . import h5io
+            path = os.path.join(dump_dir, h5io.file_name(simname, "full", None))
+            h5io.write_full(path, self, time=t)
+        raise DivergenceError(t, d, path)
 
     def diag(self) -> np.ndarray:
         out = np.zeros(3)
         check(lib().sparc_hip_diag(self.h, dptr(out)), "sparc_hip_diag")
         return out
 
-    def run(self, n0: int, n1: int, check_every: int = 1) -> np.ndarray:
-        """Driver loop nn = n0..n1 with the reference's failure guards
-        evaluated every ``check_every`` steps (driver.f90:175-196)."""
+    def run(self, n0: int, n1: int, check_every: int = 1,
+            dump_dir: Optional[str] = None, simname: str = "") -> np.ndarray:
+        """Driver loop nn = n0..n1 returning the diagnostics (norm_KE,
+        min rho+rho0, min p+p0) of every ``check_every``-th step (a
+        synchronisation each); the failure checks run on the device every
+        step and a failure raises DivergenceError (driver.f90:175-196), after
+        writing the dump when ``dump_dir`` is given."""
         hist = []
         for nn in range(n0, n1 + 1):
             self.step(nn)
             if (nn - n0) % check_every == 0 or nn == n1:
-                d = self.diag()
-                hist.append(d)
-                if not (d[0] <= 1.0e3) or d[1] <= 0.0 or d[2] <= 0.0:
-                    raise DivergenceError(f"step {nn}: norm_KE={d[0]:.5e} min(rho+rho0)={d[1]:.5e} "
-                                          f"min(p+p0)={d[2]:.5e}")
+                self.check_guard(dump_dir, simname)
+                hist.append(self.diag())
             if nn != 0:
                 self.filter_vars(nn)
+        self.check_guard(dump_dir, simname)
         return np.array(hist)
 
     def sync(self):
