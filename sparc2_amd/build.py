@@ -93,6 +93,45 @@ def build_fits(verbose: bool = True):
     return FITSLIB
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+ROOT = os.path.dirname(HERE)
+SHIM_TEST = os.path.join(ROOT, "tests", "fortran", "_build", "drive_quiet")
+
+
+def build_fortran_shim_test(verbose: bool = True):
+    """The test program of the Fortran drop-in shim (fortran/sparc_gpu.f90
+    over fortran/sparc_hip.f90, with the test's stand-in all_modules) linked
+    against libsparc_hip.so; None when flang is absent."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print(f"flang not found at {FLANG}: shim test program not built")
+        return None
+    out = os.path.dirname(SHIM_TEST)
+    os.makedirs(out, exist_ok=True)
+    srcs = [os.path.join(HERE, "fortran", "sparc_hip.f90"),
+            os.path.join(ROOT, "tests", "fortran", "all_modules.f90"),
+            os.path.join(HERE, "fortran", "sparc_gpu.f90"),
+            os.path.join(ROOT, "tests", "fortran", "drive_quiet.f90")]
+    objs = []
+    for src in srcs:
+        obj = os.path.join(out, os.path.splitext(os.path.basename(src))[0] + ".o")
+        r = subprocess.run([FLANG, "-c", "-O2", "-fPIC", "-module-dir", out, src, "-o", obj],
+                           capture_output=True, text=True, cwd=out)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed for {src}:\n{r.stderr}")
+        objs.append(obj)
+    tmp = SHIM_TEST + ".tmp"
+    r = subprocess.run([FLANG, *objs, f"-L{HERE}", "-lsparc_hip",
+                        "-Wl,-rpath,$ORIGIN/../../../sparc2_amd", "-o", tmp],
+                       capture_output=True, text=True, cwd=out)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang link failed:\n{r.stderr}")
+    os.replace(tmp, SHIM_TEST)
+    if verbose:
+        print(f"built {SHIM_TEST}")
+    return SHIM_TEST
+
+
 def build(jobs: int = 4, verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
@@ -107,6 +146,7 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
         print(f"built {LIB}")
     build_h5(verbose)
     build_fits(verbose)
+    build_fortran_shim_test(verbose)
     return LIB
 
 

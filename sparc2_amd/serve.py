@@ -95,10 +95,13 @@ def cfl_timestep(cfg: RunConfig, bg, vmax_alfven=1.0e7, courant=1.0, quantum=0.0
 def poisson_stream(n: int, rate: float = 4.0, seed: int = 7, precision: str = "fp32",
                    grids=((64, 64, 64), (128, 128, 128), (192, 192, 160), (256, 256, 192),
                           (384, 384, 256)),
-                   steps=(500, 5000)) -> List[Request]:
+                   steps=(500, 5000), pulse_amp: float = 1.0e2) -> List[Request]:
     """Seeded request stream: Poisson arrivals, grids drawn from `grids`,
-    B0 uniform in [0, 3] kG (0 -> quiet), random source position in the
-    central half, step count uniform in `steps`."""
+    B0 uniform in [0, 3] kG (below 1 G the quiet kernel), random source
+    position in the central half, step count uniform in `steps`.  The pulse
+    is 1 m/s by default (the equations are linear; at 100 m/s the longer
+    flux-tube runs drive p + p0 negative in their tenuous top layers, which
+    the reference's failure checks stop)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -106,7 +109,7 @@ def poisson_stream(n: int, rate: float = 4.0, seed: int = 7, precision: str = "f
         t += rng.exponential(1.0 / rate)
         nx, ny, nz = grids[rng.integers(len(grids))]
         B0 = float(rng.uniform(0.0, 3000.0))
-        mhd = B0 > 1.0
+        mhd = B0 >= 1.0
         L = 100.0e8 if mhd else 50.0e8
         px, py = rng.uniform(0.25 * L, 0.75 * L, size=2)
         zr = (-20.0, 1.0) if mhd else (-10.0, 0.5)
@@ -118,7 +121,7 @@ def poisson_stream(n: int, rate: float = 4.0, seed: int = 7, precision: str = "f
             pmlc=((9.2103, 2.0, 8.0, 0.1),) * 6, t_xy=30.0, t_z=10.0,
             pulse_t0=1000.0 / 3.0, pulse_t1=450.0, pulse_st=150.0,
             pulse_px=float(px), pulse_py=float(py), pulse_pz=-0.15e8,
-            pulse_sr=1.0e8, pulse_sz=1.0e8, pulse_amp=1.0e4, maxva=1.0e7, precision=precision)
+            pulse_sr=1.0e8, pulse_sz=1.0e8, pulse_amp=pulse_amp, maxva=1.0e7, precision=precision)
         out.append(Request(rid=rid, arrival=t, cfg=cfg, zrange=zr, nsteps=nsteps, B0=B0))
     return out
 

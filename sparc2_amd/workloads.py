@@ -76,14 +76,17 @@ def build(name: str, **kw):
     return cfg, bg
 
 
-def batch_population(n=64, seed=2024, precision="fp32"):
+def batch_population(n=64, seed=2024, precision="fp32", pulse_amp=1.0e2):
     """The batched workload: n independent runs drawn with a seeded generator
-    -- nx, ny from {48, 64, 96, 128, 192}, nz uniform in 96..256, B0 from
-    {0, 0.5, ..., 3.0} kG (0 gives the quiet kernel, otherwise the flux-tube
-    MHD run on sunspot_mhd's 100 Mm box with 10-cell CPML, kappa 8, on all
-    faces), the
-    source uniform in the central half of the box, dt from the CFL estimate.
-    Returns [(cfg, raw background)]."""
+    -- nx, ny from {48, 64, 96, 128, 192}, nz uniform in 96..256, B0 uniform
+    in [0, 3] kG (below 1 G the quiet kernel, otherwise the flux-tube MHD run
+    on sunspot_mhd's 100 Mm box with 10-cell CPML, kappa 8, on all faces),
+    the source uniform in the central half of the box, dt from the CFL
+    estimate.  The pulse amplitude defaults to 1 m/s: the equations are
+    linear, so it only scales the fields, and at the reference's 100 m/s the
+    wave reaching the tenuous top layers of the flux-tube atmospheres drives
+    p + p0 negative within ~700 steps, which the reference's failure checks
+    (driver.f90:175-183) stop.  Returns [(cfg, raw background)]."""
     import numpy as np
 
     from .serve import cfl_timestep
@@ -93,8 +96,8 @@ def batch_population(n=64, seed=2024, precision="fp32"):
     for _ in range(n):
         nx, ny = (int(sizes[rng.integers(len(sizes))]) for _ in range(2))
         nz = int(rng.integers(96, 257))
-        B0 = 500.0 * int(rng.integers(0, 7))
-        mhd = B0 > 0.0
+        B0 = float(rng.uniform(0.0, 3000.0))
+        mhd = B0 >= 1.0
         L = 100.0e8 if mhd else 50.0e8
         px, py = rng.uniform(0.25 * L, 0.75 * L, size=2)
         if mhd:
@@ -103,6 +106,7 @@ def batch_population(n=64, seed=2024, precision="fp32"):
         else:
             cfg, zr = quiet_acoustic(nx=nx, ny=ny, nz=nz, precision=precision)
         cfg.pulse_px, cfg.pulse_py = float(px), float(py)
+        cfg.pulse_amp = float(pulse_amp)
         cfg.__post_init__()
         bg = make_background(cfg, *zr, B0=B0, R0=4.0e8) if mhd else make_background(cfg, *zr)
         cfg.timestep = cfl_timestep(cfg, bg)

@@ -44,7 +44,11 @@ def background(cfg, zbot=None, ztop=None):
 
 
 def random_state(cfg, seed=1, amp=1e-4):
-    """Seeded smooth-ish perturbation state and PML bases."""
+    """Seeded smooth-ish perturbation state and PML bases.  Each field is
+    scaled by its local background (rho by rho0, p by p0, v by the sound
+    speed, b by sqrt(rho0) c, all in the units of the background at cell
+    (1,1,1)), so rho+rho0 and p+p0 stay positive in every layer and the
+    reference's failure checks (driver.f90:175-183) pass."""
     rng = np.random.default_rng(seed)
     shape = (cfg.nv, cfg.nz, cfg.ny, cfg.nx)
     a = rng.standard_normal(shape)
@@ -52,7 +56,14 @@ def random_state(cfg, seed=1, amp=1e-4):
     for ax in (1, 2, 3):
         for _ in range(2):
             a = 0.25 * np.roll(a, 1, ax) + 0.5 * a + 0.25 * np.roll(a, -1, ax)
-    a *= amp
+    bg = background(cfg)
+    r0, c0 = bg["rho0"][0, 0, 0], bg["cs0"][0, 0, 0]
+    rho = bg["rho0"] / r0
+    c = bg["cs0"] / c0
+    p = bg["pre0"] / (r0 * c0 * c0)
+    scale = [rho, c, c, c, p] + [np.sqrt(rho) * c] * 3
+    for q in range(cfg.nv):
+        a[q] *= amp * scale[q]
     psi = {}
     for f in range(6):
         if cfg.face_active(f):

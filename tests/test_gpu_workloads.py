@@ -198,13 +198,13 @@ def test_batch_population_equals_alone():
 
 
 def test_batch_population_run_fp32_after_2000_steps():
-    """One population run in FP32 (batched with two others) against the same
-    run alone in FP64 (which equals the oracle to 1e-11) after 2000 steps."""
+    """The three smallest population runs in FP32, batched, against the
+    same runs alone in FP64 (which equal the oracle to 1e-11) after 2000
+    steps: relative L2 <= 1e-5 per field."""
     from sparc2_amd.batch import Batch
     from sparc2_amd.solver import Solver
     pop = _population()
-    small = sorted(range(64), key=lambda q: pop[q][0].npoints)
-    pick = [q for q in small if pop[q][0].mhd][:2] + [q for q in small if not pop[q][0].mhd][:1]
+    pick = sorted(range(64), key=lambda q: pop[q][0].npoints)[:3]
     sol = [Solver(pop[q][0], pop[q][1]) for q in pick]
     b = Batch(sol, use_graph=True)
     b.advance(2000)
