@@ -173,6 +173,18 @@ int sparc_hip_filtx(sparc_ctx* ctx, const double* in, double* out);
 int sparc_hip_filty(sparc_ctx* ctx, const double* in, double* out);
 int sparc_hip_filtz(sparc_ctx* ctx, const double* in, double* out);
 
+/* ddx / ddy / ddz (axis 0, 1, 2) without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  It runs the kernel of sparc_hip_ddx/ddy/ddz with
+ * the host input and output staged through a static device array the
+ * library holds on every device it runs on (64 MiB each, until the process
+ * ends) instead of the context's operator buffer, which those entry points
+ * allocate on their first call.  Limit: nx*ny*nz <= 4194304 points; larger
+ * grids are refused (SPARC_E_INVALID).  All calls on one device share that
+ * array: calls that overlap on one device race.  Synchronises. */
+int sparc_hip_deriv_noalloc(sparc_ctx* ctx, int axis, const double* in, double* out, int bc);
+
 /* Wait for all work queued on the context. */
 int sparc_hip_sync(sparc_ctx* ctx);
 

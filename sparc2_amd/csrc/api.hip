@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -333,6 +334,34 @@ const char* sparc_hip_strerror(int code) {
   }
 }
 
+// Staging of the allocation-free derivative (sparc_hip_deriv_noalloc): input
+// and output of one field in a file-scope device array, so that entry point
+// needs no hipMalloc.  Every device the library runs on holds its own copy
+// (64 MiB) until the process ends.  In host code `scratch` names a host-side
+// stand-in: its device address comes from hipGetSymbolAddress, once per
+// device, when the first context on that device is created.
+static constexpr long kNoAllocMaxPoints = 1L << 22;  // 4,194,304 grid points
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxPoints];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// With device `dev` current: look up and keep this device's scratch address.
+static int scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(SPARC_E_INVALID, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return SPARC_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(SPARC_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = p;
+  return SPARC_OK;
+}
+
 int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
   if (!cfg || !out) return fail(SPARC_E_INVALID, "null argument");
   *out = nullptr;
@@ -398,6 +427,7 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
     return code;
   };
   if (hipSetDevice(c->dev) != hipSuccess) return bail(fail(SPARC_E_HIP, "hipSetDevice failed"));
+  if ((rc = scratch_setup(c->dev))) return bail(rc);
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(SPARC_E_HIP, "hipStreamCreate failed"));
 
@@ -924,6 +954,39 @@ static int op_deriv(sparc_ctx* c, int axis, const double* in, double* out, int b
                    c->st) != hipSuccess)
     return fail(SPARC_E_HIP, "derivative launch failed");
   return op_finish(c, out);
+}
+
+// ddx/ddy/ddz without any allocation: the same kernel as op_deriv, staged
+// through this device's scratch (input at its start, output after
+// kNoAllocMaxPoints).  Every element of the output slot is written, so no
+// clearing is needed.  Calls on one device share the array: calls that
+// overlap on one device (from several threads or contexts) race.
+int sparc_hip_deriv_noalloc(sparc_ctx* c, int axis, const double* in, double* out, int bc) {
+  if (!c || !in || !out) return fail(SPARC_E_INVALID, "null argument");
+  if (axis < 0 || axis > 2) return fail(SPARC_E_INVALID, "axis must be 0 (x), 1 (y) or 2 (z)");
+  if (axis < 2 && bc != 1 && bc != 5) return fail(SPARC_E_INVALID, "bc must be 1 or 5");
+  if (axis == 2 && bc != 1) return fail(SPARC_E_INVALID, "z derivatives take bc = 1 (z is never periodic)");
+  if (c->N > kNoAllocMaxPoints)
+    return fail(SPARC_E_INVALID, "grid of " + std::to_string(c->N) + " points exceeds the " +
+                                     std::to_string(kNoAllocMaxPoints) +
+                                     "-point staging of sparc_hip_deriv_noalloc; use sparc_hip_ddx/ddy/ddz");
+  if (c->dev < 0 || c->dev >= kMaxDevices) return fail(SPARC_E_INVALID, "device ordinal beyond the scratch table");
+  void* p = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    p = g_scratch_ptr[c->dev];
+  }
+  if (!p) return fail(SPARC_E_STATE, "scratch address of the device was not set up");
+  HIPCHK(hipSetDevice(c->dev));
+  double* s_in = (double*)p;
+  double* s_out = s_in + kNoAllocMaxPoints;
+  HIPCHK(hipMemcpyAsync(s_in, in, sizeof(double) * c->N, hipMemcpyHostToDevice, c->st));
+  const double sxy = axis == 0 ? c->cfg.stretchx : c->cfg.stretchy;
+  if (launch_deriv(axis, s_in, s_out, c->nx, c->ny, c->nz, sxy, c->stretch_d, bc, c->st) != hipSuccess)
+    return fail(SPARC_E_HIP, "derivative launch failed");
+  HIPCHK(hipMemcpyAsync(out, s_out, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SPARC_OK;
 }
 
 int sparc_hip_ddx(sparc_ctx* c, const double* in, double* out, int bc) { return op_deriv(c, 0, in, out, bc); }

@@ -57,7 +57,7 @@ module sparc_hip
        sparc_hip_batch_advance, sparc_hip_batch_info, sparc_hip_batch_destroy, &
        sparc_hip_autotune, sparc_hip_set_tile, sparc_hip_batch_time_stage, sparc_hip_set_damping, &
        sparc_hip_set_forcing, sparc_hip_set_guard, sparc_hip_guard, sparc_hip_batch_filter, &
-       sparc_hip_stream_bench, &
+       sparc_hip_stream_bench, sparc_hip_deriv_noalloc, &
        sparc_error_message
 
   interface
@@ -253,6 +253,18 @@ module sparc_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: batch
       integer(c_int), intent(in) :: time(*)
+    end function
+
+    ! ddx/ddy/ddz (axis 0/1/2) with no device allocation (static staging,
+    ! grids of at most 4194304 points; overlapping calls on a device race)
+    integer(c_int) function sparc_hip_deriv_noalloc(ctx, axis, input, output, bc) &
+         bind(C, name="sparc_hip_deriv_noalloc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: axis
+      real(c_double), intent(in) :: input(*)
+      real(c_double), intent(out) :: output(*)
+      integer(c_int), value :: bc
     end function
 
     ! bandwidth probe: nread arrays of n doubles summed into one more
