@@ -161,96 +161,110 @@ def mode_precision(args):
         "max_rel_l2": max(max(v.values()) for v in l2.values())}), flush=True)
 
 
-def mode_batch(args):
-    """The batched form: a seeded population of independent runs (quiet and
-    MHD, FP32+Kahan) advanced together; aggregate point-updates per second."""
+def _batch_run(pop, precision, warmup, steps, autotune):
+    """Advance the population batched in one precision; returns GPt/s,
+    ms/step, the tuned groups, launch sizes and the runs that failed the
+    reference's checks (their points are not counted)."""
     from sparc2_amd.batch import Batch
     from sparc2_amd.solver import Solver
-    from sparc2_amd.workloads import batch_population
     t0 = time.perf_counter()
-    pop = batch_population(args.runs or 64)
-    sol = [Solver(c, b) for c, b in pop]
-    log(f"[batch] {len(sol)} runs set up in {time.perf_counter() - t0:.1f} s")
+    sol = [Solver(c.replace(precision=precision), b) for c, b in pop]
+    log(f"[batch {precision}] {len(sol)} runs set up in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    b = Batch(sol, use_graph=True, autotune=not args.no_autotune)
-    tiles = sorted({s.tile()[0][:2] for s in sol})
-    log(f"[batch] autotuned in {time.perf_counter() - t0:.1f} s: tile shapes {tiles}")
-    b.advance(args.warmup)
+    b = Batch(sol, use_graph=True, autotune=autotune)
+    log(f"[batch {precision}] batch created (autotune {autotune}) in {time.perf_counter() - t0:.1f} s")
+    b.advance(warmup, check_guard=False)
     for s in sol:
         s.sync()
-    K = args.steps
     t0 = time.perf_counter()
-    b.advance(K, check_guard=False)
+    b.advance(steps, check_guard=False)
     for s in sol:
         s.sync()
     el = time.perf_counter() - t0
     failed = b.failures()
-    if failed:
-        raise RuntimeError(f"runs failed the reference's checks in the timed steps: {failed}")
-    pts = sum(c.npoints for c, _ in pop)
+    ok = [q for q in range(len(pop)) if q not in failed]
+    pts = sum(pop[q][0].npoints for q in ok)
+    w = 4 if precision == "fp32" else 8
+    alg = sum(alg_bytes_step(pop[q][0].replace(precision=precision), w) for q in ok)
     groups, blocks = b.info()
-    n_mhd = sum(1 for c, _ in pop if c.mhd)
+    tiles = ([[int(k[0]), int(k[1]), *v] for k, v in sorted(b.tiles.items())] if b.tiles else None)
     b.close()
-    plain = None
-    if args.fp64_steps > 0:
-        # the same runs, one at a time, plain FP64 (the reference's own path)
-        for s in sol:
-            s.close()
-        tot = 0.0
-        for c, bgr in pop:
-            s = Solver(c.replace(precision="fp64"), bgr)
-            s.advance(0, 1)
-            s.sync()
-            t1 = time.perf_counter()
-            s.advance(2, 1 + args.fp64_steps)
-            s.sync()
-            tot += time.perf_counter() - t1
-            s.close()
-        plain = pts * args.fp64_steps / tot / 1e9
-        log(f"[batch] plain FP64, runs one at a time: {plain:.4f} GPt/s")
+    for s in sol:
+        s.close()
+    if failed:
+        log(f"[batch {precision}] runs stopped by the failure checks (not counted): "
+            f"{ {q: str(e) for q, e in failed.items()} }")
+    return {"gpts": pts * steps / el / 1e9, "ms_per_step": el / steps * 1e3,
+            "alg_GBps": alg * steps / el / 1e9, "groups": groups, "launch_blocks": blocks,
+            "tuned_groups": tiles, "failed_runs": sorted(failed)}
+
+
+def mode_batch(args):
+    """The batched form: the seeded population of independent runs advanced
+    together in FP32 with the compensated update, and the same population
+    batched in plain FP64 (its twin); aggregate point-updates per second and
+    the algorithmic HBM fraction of each."""
+    from sparc2_amd.workloads import batch_population
+    pop = batch_population(args.runs or 64)
+    K = args.steps
+    f32 = _batch_run(pop, "fp32", args.warmup, K, not args.no_autotune)
+    f64 = _batch_run(pop, "fp64", args.warmup, K, not args.no_autotune)
+    ceiling = stream_ceiling()
+    for r in (f32, f64):
+        r["hbm_frac"] = round(r["alg_GBps"] / HBM_PEAK_GBS, 4)
+        r["frac_of_stream_ceiling_4:1"] = round(r["alg_GBps"] / ceiling["4:1"], 4)
+    n_mhd = sum(1 for c, _ in pop if c.mhd)
     print(json.dumps({
-        "metric": METRIC, "mode": "batch", "value": round(pts * K / el / 1e9, 5),
+        "metric": METRIC, "mode": "batch", "value": round(f32["gpts"], 5),
         "unit": "GPt-updates/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
-        "ms_per_step": round(el / K * 1e3, 4), "higher_is_better": True, "dtype": "f32+kahan",
+        "ms_per_step": round(f32["ms_per_step"], 4), "higher_is_better": True,
+        "dtype": "f32+kahan",
         "data": "synthetic (seeded population: polytrope / flux-tube backgrounds)",
         "config": {"workload": "batch_population", "runs": len(pop), "mhd_runs": n_mhd,
-                   "points": pts, "groups": groups, "launch_blocks": blocks, "graph": True,
-                   "autotuned_tiles": [list(t) for t in tiles],
-                   # per variant group (MHD, 3-D backgrounds): tile in lanes, z-chunk multiple
-                   "tuned_groups": ([[int(k[0]), int(k[1]), *v] for k, v in sorted(b.tiles.items())]
-                                    if b.tiles else None)},
-        "plain_fp64_gpts": round(plain, 5) if plain else None,
-        "speedup_vs_plain_fp64": round(pts * K / el / 1e9 / plain, 3) if plain else None}),
-          flush=True)
+                   "points": sum(c.npoints for c, _ in pop), "graph": True,
+                   "autotune": not args.no_autotune},
+        "roofline": {"bound": "hbm", "achieved": round(f32["alg_GBps"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": f32["hbm_frac"],
+                     "frac_of_stream_ceiling_4:1": f32["frac_of_stream_ceiling_4:1"],
+                     "stream_ceiling_GBps": ceiling,
+                     "basis": "algorithmic bytes per step of every counted run x steps / time"},
+        "fp32": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in f32.items()},
+        "fp64_twin": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in f64.items()},
+        "speedup_vs_fp64_batched": round(f32["gpts"] / f64["gpts"], 3)}), flush=True)
 
 
 def mode_serve(args):
     """The served form: a seeded Poisson stream of requests through the
-    continuous-batching server, FP32+Kahan and then the same stream in plain
-    FP64; runs/h, throughput and latency percentiles of each.  --step-scale
+    continuous-batching server, in each precision of --serve-precisions;
+    runs/h, throughput and latency percentiles of each.  --step-scale
     shrinks the drawn step counts (a bounded sample of the stream)."""
     from sparc2_amd.serve import Server, poisson_stream
     n = args.runs or 8
     lo, hi = (max(1, int(round(v * args.step_scale))) for v in (500, 5000))
     out = {}
-    for prec in ("fp32", "fp64"):
+    for prec in args.serve_precisions.split(","):
         reqs = poisson_stream(n, precision=prec, steps=(lo, hi))
-        r = Server().run(reqs, verbose=True)
+        r = Server(autotune=not args.no_autotune).run(reqs, verbose=True)
+        failed = [rid for rid, res in r["results"].items() if res.error is not None]
         out[prec] = {"runs_per_h": round(r["runs_per_h"], 2), "gpts": round(r["gpts"], 5),
                      "gpts_busy": round(r["gpts_busy"], 5), "wall_s": round(r["wall_s"], 3),
+                     "rebuild_s": round(r["rebuild_s"], 3),
                      "latency_p50_s": round(r["latency_p50"], 3),
                      "latency_p95_s": round(r["latency_p95"], 3),
-                     "latency_p99_s": round(r["latency_p99"], 3)}
+                     "latency_p99_s": round(r["latency_p99"], 3), "failed_requests": failed}
         log(f"[serve] {prec}: {out[prec]}")
-    print(json.dumps({
-        "metric": METRIC, "mode": "serve", "value": out["fp32"]["runs_per_h"], "unit": "runs/h",
-        "n_gpus": 1, "higher_is_better": True, "dtype": "f32+kahan",
+    first = list(out)[0]
+    line = {
+        "metric": METRIC, "mode": "serve", "value": out[first]["runs_per_h"], "unit": "runs/h",
+        "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32+kahan" if first == "fp32" else "f64",
         "data": "synthetic (seeded Poisson stream, rate 4/s)",
         "config": {"workload": "poisson_stream", "requests": n, "steps_drawn_from": [lo, hi],
-                   "grids": "64^3 .. 384x384x256"},
-        "fp32": out["fp32"], "fp64": out["fp64"],
-        "gpts_ratio_fp32_fp64": round(out["fp32"]["gpts_busy"] / out["fp64"]["gpts_busy"], 3)}),
-          flush=True)
+                   "grids": "64^3 .. 384x384x256", "autotune": not args.no_autotune}}
+    line.update(out)
+    if "fp32" in out and "fp64" in out:
+        line["gpts_ratio_fp32_fp64"] = round(out["fp32"]["gpts_busy"] / out["fp64"]["gpts_busy"], 3)
+    print(json.dumps(line), flush=True)
 
 
 def mode_profile(args):
@@ -352,8 +366,8 @@ def main():
                          "one batch; serve: a Poisson request stream; profile: a 1 GiB copy "
                          "(the HBM-counter calibration) then --steps driver steps, for rocprofv3")
     ap.add_argument("--no-autotune", action="store_true", help="batch: keep the default tiles")
-    ap.add_argument("--fp64-steps", type=int, default=0,
-                    help="batch: also time this many steps of every run alone in plain FP64")
+    ap.add_argument("--serve-precisions", default="fp32,fp64",
+                    help="serve: precisions to run the stream in, comma separated")
     ap.add_argument("--step-scale", type=float, default=1.0,
                     help="serve: scale of the drawn step counts (500..5000 at 1)")
     ap.add_argument("--runs", type=int, default=None,

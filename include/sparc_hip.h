@@ -258,6 +258,10 @@ int sparc_hip_batch_time_stage(sparc_batch* b, int reps, double* ms_out);
  * ratios as well as against the data-sheet HBM peak. */
 int sparc_hip_stream_bench(int device, long n, int nread, int reps, double* gbps);
 
+/* Free and total device memory in bytes (hipMemGetInfo): the served form
+ * sizes its HBM budget with it.  No counterpart in the reference. */
+int sparc_hip_device_memory(int device, long* free_bytes, long* total_bytes);
+
 const char* sparc_hip_strerror(int code);
 const char* sparc_hip_last_error(void);
 int sparc_hip_version(void);

@@ -63,7 +63,7 @@ EXPORTS = [
     "sparc_hip_batch_info", "sparc_hip_batch_destroy", "sparc_hip_alfven_search",
     "sparc_hip_get_slab", "sparc_hip_autotune", "sparc_hip_set_tile", "sparc_hip_batch_time_stage",
     "sparc_hip_set_guard", "sparc_hip_guard", "sparc_hip_batch_filter", "sparc_hip_stream_bench",
-    "sparc_hip_deriv_noalloc",
+    "sparc_hip_deriv_noalloc", "sparc_hip_device_memory",
 ]
 
 _lib = None
@@ -109,6 +109,7 @@ def lib() -> C.CDLL:
     L.sparc_hip_set_guard.argtypes = [vp, i]
     L.sparc_hip_stream_bench.argtypes = [i, C.c_long, i, i, DP]
     L.sparc_hip_deriv_noalloc.argtypes = [vp, i, DP, DP, i]
+    L.sparc_hip_device_memory.argtypes = [i, C.POINTER(C.c_long), C.POINTER(C.c_long)]
     L.sparc_hip_guard.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_long), C.POINTER(C.c_long), DP]
     L.sparc_hip_batch_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_long)]
     L.sparc_hip_batch_destroy.argtypes = [vp]

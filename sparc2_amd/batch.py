@@ -52,6 +52,12 @@ def _apply(members: Sequence[Solver], tile, auto, mult):
         s.set_tile((tile[0], tile[1], 0 if mult == 1 else min(k * mult, s.cfg.nz)))
 
 
+# Tuned choice per variant group, keyed by the group's variant and the
+# sorted tile keys of its runs: a group that comes back (the served form
+# rebuilds its batch whenever runs join or leave) is not timed again.
+_GROUP_CACHE: Dict[tuple, tuple] = {}
+
+
 def tune_tiles(solvers: Sequence[Solver], reps: int = 3) -> Dict[tuple, tuple]:
     """Tile autotuning for a batch.  Runs that share a kernel variant share
     one launch, so the shape is chosen per variant group, by timing a middle
@@ -65,6 +71,15 @@ def tune_tiles(solvers: Sequence[Solver], reps: int = 3) -> Dict[tuple, tuple]:
         groups.setdefault((s.cfg.magnetic == 2, bool(s.st.bg3d)), []).append(s)
     chosen = {}
     for key, members in groups.items():
+        ckey = (key, tuple(sorted(s.tile_key() for s in members)))
+        if ckey in _GROUP_CACHE:
+            t, mult = _GROUP_CACHE[ckey]
+            for s in members:
+                s.set_tile((t[0], t[1], 0))
+            auto = [s.tile()[0][2] for s in members]
+            _apply(members, t, auto, mult)
+            chosen[key] = (t[0], t[1], mult)
+            continue
         best, best_ms, best_auto = None, float("inf"), None
         for t in TILES:
             try:
@@ -88,6 +103,7 @@ def tune_tiles(solvers: Sequence[Solver], reps: int = 3) -> Dict[tuple, tuple]:
             raise SparcError(f"no tile shape fits every grid of variant {key}")
         _apply(members, best[:2], best_auto, best[2])
         chosen[key] = best
+        _GROUP_CACHE[ckey] = (best[:2], best[2])
     return chosen
 
 

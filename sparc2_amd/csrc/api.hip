@@ -291,8 +291,11 @@ static void set_geometry(sparc_ctx* c, int tx, int ty, int kc) {
     // 1.41 ms at 2048 blocks, 1.30 at 1024, 1.23 at 512); the MHD xy kernel
     // has no queue and runs fastest with many shallow chunks (4 planes,
     // measured on 256^3: 16x16 FP64 1.85 ms at 8 planes, 1.77 ms at 4)
+    // small grids: 2-plane chunks (64x64x128 quiet, one step: FP64 32x8
+    // 0.192 ms at 4 planes, 0.184 at 2, 0.198 at 1; FP32 0.171 / 0.119 /
+    // 0.120), so a launch still has ~512 blocks
     long want = c->zp ? 16384 : 512;
-    kc = (int)std::max<long>(4, (c->nz * tiles + want - 1) / want);
+    kc = (int)std::max<long>(c->zp ? 4 : 2, (c->nz * tiles + want - 1) / want);
     kc = std::min(kc, c->nz);
   }
   c->kc = kc;
@@ -1104,11 +1107,12 @@ int sparc_hip_autotune(sparc_ctx* c, int reps, int tile_out[3], double* ms_out) 
     int rc = retile(c, sh[0], sh[1], 0);
     if (rc) return rc;
     const int kauto = c->kc;
-    // the automatic z chunk, half of it (more blocks) and twice it
-    const int kcs[3] = {kauto, std::max(4, kauto / 2), std::min(c->nz, kauto * 2)};
-    for (int q = 0; q < 3; ++q) {
+    // the automatic z chunk, a half and a quarter of it (more blocks: small
+    // grids run fastest at 1-2 planes) and twice it
+    const int kcs[4] = {kauto, std::max(1, kauto / 2), std::max(1, kauto / 4), std::min(c->nz, kauto * 2)};
+    for (int q = 0; q < 4; ++q) {
       const int kc = kcs[q];
-      if (q > 0 && kc == kauto) continue;
+      if (q > 0 && (kc == kauto || kc == kcs[q - 1])) continue;
       if ((rc = retile(c, sh[0], sh[1], kc))) return rc;
       double ms = 0.0;
       rc = c->f32 ? time_middle_stage<float>(c, reps, &ms) : time_middle_stage<double>(c, reps, &ms);
@@ -1644,5 +1648,16 @@ int sparc_hip_stream_bench(int device, long n, int nread, int reps, double* gbps
   cleanup();
   if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("stream probe: ") + hipGetErrorString(e));
   *gbps = (double)(nread + 1) * 8.0 * (double)n / (ms * 1e-3 / reps) / 1e9;
+  return SPARC_OK;
+}
+
+// Device memory (free, total) in bytes: the served form's HBM budget.
+int sparc_hip_device_memory(int device, long* free_bytes, long* total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(SPARC_E_INVALID, "null argument");
+  HIPCHK(hipSetDevice(device));
+  size_t f = 0, t = 0;
+  HIPCHK(hipMemGetInfo(&f, &t));
+  *free_bytes = (long)f;
+  *total_bytes = (long)t;
   return SPARC_OK;
 }

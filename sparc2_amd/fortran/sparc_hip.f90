@@ -57,7 +57,7 @@ module sparc_hip
        sparc_hip_batch_advance, sparc_hip_batch_info, sparc_hip_batch_destroy, &
        sparc_hip_autotune, sparc_hip_set_tile, sparc_hip_batch_time_stage, sparc_hip_set_damping, &
        sparc_hip_set_forcing, sparc_hip_set_guard, sparc_hip_guard, sparc_hip_batch_filter, &
-       sparc_hip_stream_bench, sparc_hip_deriv_noalloc, &
+       sparc_hip_stream_bench, sparc_hip_deriv_noalloc, sparc_hip_device_memory, &
        sparc_error_message
 
   interface
@@ -265,6 +265,13 @@ module sparc_hip
       real(c_double), intent(in) :: input(*)
       real(c_double), intent(out) :: output(*)
       integer(c_int), value :: bc
+    end function
+
+    integer(c_int) function sparc_hip_device_memory(device, free_bytes, total_bytes) &
+         bind(C, name="sparc_hip_device_memory")
+      import :: c_int, c_long
+      integer(c_int), value :: device
+      integer(c_long), intent(out) :: free_bytes, total_bytes
     end function
 
     ! bandwidth probe: nread arrays of n doubles summed into one more

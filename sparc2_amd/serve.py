@@ -74,6 +74,15 @@ def device_footprint(cfg: RunConfig) -> int:
     return w * (n_state * cfg.nv * fs + n_bg * fs + psi * (3 + 2 * cfg.npsi)) + (1 << 20)
 
 
+def device_memory(device: int = 0):
+    """(free, total) bytes of the device (hipMemGetInfo)."""
+    import ctypes as C
+    from ._lib import check, lib
+    f, t = C.c_long(), C.c_long()
+    check(lib().sparc_hip_device_memory(int(device), C.byref(f), C.byref(t)), "sparc_hip_device_memory")
+    return f.value, t.value
+
+
 def surface_plane(zaxis_mm) -> int:
     return int(np.argmin(np.abs(np.asarray(zaxis_mm))))
 
@@ -138,10 +147,12 @@ class Server:
     layout of WRITE_OUT_CHUNK, all_modules.f90:1566-1799), named
     ``req<rid>_chunk_NNNN.h5`` with NNNN the slice count."""
 
-    def __init__(self, hbm_budget: float = 200e9, max_chunk: int = 200, use_graph: bool = True,
+    def __init__(self, hbm_budget: Optional[float] = None, max_chunk: int = 200, use_graph: bool = True,
                  device: int = 0, keep_final: bool = False, setup_workers: int = 2,
                  poll_chunk: int = 25, chunk_dir: Optional[str] = None,
+                 autotune: bool = False,
                  clock: Callable[[], float] = time.perf_counter):
+        # None: 90% of the device memory free when the stream starts
         self.budget = hbm_budget
         self.max_chunk = max_chunk
         self.use_graph = use_graph
@@ -150,6 +161,9 @@ class Server:
         self.setup_workers = max(1, setup_workers)
         self.poll_chunk = max(1, poll_chunk)
         self.chunk_dir = chunk_dir
+        # autotune: tile and z chunk tuned per variant group of each new
+        # membership (batch.tune_tiles; groups seen before reuse their choice)
+        self.autotune = autotune
         self.clock = clock
 
     def _start(self, req: Request):
@@ -164,6 +178,9 @@ class Server:
     def run(self, requests: List[Request], verbose: bool = False) -> Dict:
         from concurrent.futures import ThreadPoolExecutor
         from .batch import Batch
+        if self.budget is None:
+            free, _ = device_memory(self.device)
+            self.budget = 0.9 * free
         reqs = sorted(requests, key=lambda r: r.arrival)
         results = {r.rid: Result(rid=r.rid, arrival=r.arrival) for r in reqs}
         waiting: List[Request] = []    # arrived, set-up not started
@@ -175,6 +192,7 @@ class Server:
         nxt = 0
         batch, members = None, ()
         busy = 0.0
+        rebuild = 0.0     # batch (re)creation and tuning when membership changes
         say = (lambda m: print(m, file=sys.stderr, flush=True)) if verbose else (lambda m: None)
         pool = ThreadPoolExecutor(max_workers=self.setup_workers)
         try:
@@ -216,7 +234,9 @@ class Server:
                         batch.filter_deferred()
                         batch.close()
                     sol = [active[r]["solver"] for r in key]
-                    batch = Batch(sol, use_graph=self.use_graph)
+                    tb = self.clock()
+                    batch = Batch(sol, use_graph=self.use_graph, autotune=self.autotune)
+                    rebuild += self.clock() - tb
                     batch.time = [active[r]["done"] for r in key]
                     members = key
                 # advance to the next event of any run: a slice or its end
@@ -275,6 +295,7 @@ class Server:
             "results": results,
             "wall_s": wall,
             "busy_s": busy,
+            "rebuild_s": rebuild,
             "runs_per_h": len(reqs) / wall * 3600.0,
             "gpts": pts / wall / 1e9,
             "gpts_busy": pts / busy / 1e9 if busy > 0 else float("nan"),

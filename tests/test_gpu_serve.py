@@ -12,22 +12,23 @@ from tests.cases import mhd_cfg, quiet_cfg
 pytestmark = pytest.mark.gpu
 
 
-def _requests():
+def _requests(precision="fp32"):
     from sparc2_amd.serve import Request
     specs = [
-        (0.00, quiet_cfg(nx=32, ny=32, nz=40, precision="fp32"), (-10.0, 0.5), 140, 0.0),
-        (0.02, mhd_cfg(nx=32, ny=32, nz=40, precision="fp32"), (-5.0, 1.0), 90, 2500.0),
-        (0.04, quiet_cfg(nx=48, ny=32, nz=48, pulse_px=20e8, precision="fp32"), (-10.0, 0.5), 60, 0.0),
-        (0.30, mhd_cfg(nx=48, ny=32, nz=40, periodic=True, precision="fp32"), (-5.0, 1.0), 75, 1500.0),
+        (0.00, quiet_cfg(nx=32, ny=32, nz=40, precision=precision), (-10.0, 0.5), 140, 0.0),
+        (0.02, mhd_cfg(nx=32, ny=32, nz=40, precision=precision), (-5.0, 1.0), 90, 2500.0),
+        (0.04, quiet_cfg(nx=48, ny=32, nz=48, pulse_px=20e8, precision=precision), (-10.0, 0.5), 60, 0.0),
+        (0.30, mhd_cfg(nx=48, ny=32, nz=40, periodic=True, precision=precision), (-5.0, 1.0), 75, 1500.0),
     ]
     return [Request(rid=q, arrival=t, cfg=c, zrange=z, nsteps=n, B0=b, slice_every_s=15.0)
             for q, (t, c, z, n, b) in enumerate(specs)]
 
 
-def test_served_results_equal_alone(tmp_path):
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_served_results_equal_alone(tmp_path, precision):
     from sparc2_amd import h5io
     from sparc2_amd.serve import Server, cfl_timestep
-    reqs = _requests()
+    reqs = _requests(precision)
     chunks = str(tmp_path) if os.path.exists(h5io.LIB) else None
     mixed = Server(max_chunk=25, keep_final=True, chunk_dir=chunks).run(copy.deepcopy(reqs), verbose=True)
     for r in reqs:
