@@ -147,8 +147,6 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   auto slot_bo = [&](int s, uint32_t pk) -> uint32_t {
     return s_ok[s] ? (pk + s_off[s]) * W : Buf<T>::kOut;
   };
-  auto flux1 = [&](uint32_t g) -> V { return -UF(VZ, g) * B3(Ap->boy, g) + UF(VY, g) * B3(Ap->boz, g); };
-  auto flux2 = [&](uint32_t g) -> V { return -UF(VX, g) * B3(Ap->boz, g) + UF(VZ, g) * B3(Ap->box, g); };
 
   V(*L)[LP] = LB[0];  // the plane buffer in use
   // SPF: staging values of the next plane, held in registers (MHD: v, p,
@@ -323,15 +321,22 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       dst[5] = -vx * B3b(Ap->boz, bo) + vz * B3b(Ap->box, bo);
     }
   };
+  // The window stops moving at plane nz-6, where it holds planes nz-11 ..
+  // nz-1: the one-sided rows of the top five planes read exactly those, so
+  // they come from registers too.  A chunk that starts above nz-6 fills the
+  // window at that position.  (Loading the 11 rows per top plane from memory
+  // instead costs 11 dependent round trips per plane, which set the length
+  // of a whole launch when the top chunk is the last block to finish.)
+  const int kq = min(k0, nz - 6);
   if constexpr (!ZP) {
 #pragma unroll
     for (int m = 0; m < 10; ++m) {
       V t[NQ];
-      fetch(k0 - 5 + m, t);
+      fetch(kq - 5 + m, t);
 #pragma unroll
       for (int f = 0; f < NQ; ++f) q[f][m] = t[f];
     }
-    fetch(k0 + 5, nq);
+    fetch(kq + 5, nq);
   }
 
   double ke = 0.0, mr = 1.0e300, mp = 1.0e300;
@@ -442,9 +447,11 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     // ---- z derivatives (derivatives.f90:481-487,509-518,536-545) ----
     V dz[NQ];
     if constexpr (!ZP) {
+      if (k <= nz - 6 || k == k0) {
 #pragma unroll
-      for (int f = 0; f < NQ; ++f) q[f][10] = nq[f];
-      fetch(k + 6, nq);
+        for (int f = 0; f < NQ; ++f) q[f][10] = nq[f];
+      }
+      if (k < nz - 6) fetch(k + 6, nq);
       if (active) {
         const T st = ((CPtr<T>)Ap->stretch)[k];
         T stl = 0;
@@ -463,25 +470,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
             dz[f] = scl(st, stl, acc);
           }
         } else {
-          // top block: mirrored one-sided rows over planes nz-1 .. nz-11
+          // top block: mirrored one-sided rows over planes nz-1 .. nz-11,
+          // held in the window as q[f][10] .. q[f][0]
           const int r = nz - 1 - k;
-          V acc[NQ];
 #pragma unroll
-          for (int f = 0; f < NQ; ++f) acc[f] = 0;
-          for (int m = 0; m < 11; ++m) {
-            const uint32_t g = ij + (uint32_t)(nz - 1 - m) * nxy;
-            const T c = Coef<T>::b(r, m);
-            acc[0] += UF(VZ, g) * c;
-            acc[1] += UF(P, g) * c;
-            if constexpr (MHD) {
-              acc[2] += UF(BX, g) * c;
-              acc[3] += UF(BY, g) * c;
-              acc[4] += flux1(g) * c;
-              acc[5] += flux2(g) * c;
-            }
+          for (int f = 0; f < NQ; ++f) {
+            V acc = 0;
+#pragma unroll
+            for (int m = 0; m < 11; ++m) acc += q[f][10 - m] * Coef<T>::b(r, m);
+            dz[f] = -scl(st, stl, acc);
           }
-#pragma unroll
-          for (int f = 0; f < NQ; ++f) dz[f] = -scl(st, stl, acc[f]);
         }
       }
     }
@@ -495,7 +493,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       const int cc = (ty + 5) * LX + tx + HX;
       const V vx = L[0][cc], vy = L[1][cc];
       const V rho = PPF ? pc[0] : (MPX ? pm[0] : UF(RHO, g));
-      const V vz = ZP ? (MPX ? pm[1] : UF(VZ, g)) : q[0][5], p = ZP ? L[2][cc] : q[1][5];
+      const V vz = ZP ? (MPX ? pm[1] : UF(VZ, g)) : (k <= nz - 6 ? q[0][5] : UF(VZ, g)), p = L[2][cc];
       // z pass output, in CPML family order: vz, p, bx, by, F2, F1
       auto DZ = [&](int f) -> V { return MPX ? pm[2 + f] : ldv<V>(Buf<T>(Ap->dz[f], fs), g * W); };
       V dvzdz = ZP ? DZ(0) : dz[0], gpz = ZP ? DZ(1) : dz[1];
@@ -643,7 +641,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         const V red = MPX ? PMB(3) : B3(Ap->red, g);
         const V b0x = MPX ? PMB(4) : B3(Ap->box, g), b0y = MPX ? PMB(5) : B3(Ap->boy, g),
                 b0z = MPX ? PMB(6) : B3(Ap->boz, g);
-        const V bx = ZP ? L[8][cc] : q[2][5], by = ZP ? L[6][cc] : q[3][5];
+        const V bx = L[8][cc], by = L[6][cc];
         // RHS.f90:548-580
         K[BX] = red * (dyf3 - dzf2);
         K[BY] = red * (dzf1 - dxf3);
@@ -744,10 +742,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     if constexpr (NB == 1) __syncthreads();
 
     if constexpr (!ZP) {
+      if (k < nz - 6) {
 #pragma unroll
-      for (int f = 0; f < NQ; ++f)
+        for (int f = 0; f < NQ; ++f)
 #pragma unroll
-        for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
+          for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
+      }
     }
   }
 
@@ -810,8 +810,6 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
   const T dt = Ap->dt, bnext = Ap->beta_next;
   constexpr uint32_t W = sizeof(T);
 
-  auto UF = [&](int f, uint32_t g) -> T { return Buf<T>(Ap->u[f], fs)[g]; };
-  auto B3 = [&](const T* p, uint32_t g) -> T { return Buf<T>(p, fs)[g]; };
   // a column's values are read once (the queue), and the results written
   // once: all non-temporal (256^3 MHD z pass 0.481 -> 0.477 ms FP64)
   auto UFs = [&](int f, uint32_t bo) -> T { return ldv<T, kStream>(Buf<T>(Ap->u[f], fs), bo); };
@@ -829,32 +827,27 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
     dst[4] = -vx * b0z + vz * b0x;  // F2
     dst[5] = -vz * b0y + vy * b0z;  // F1
   };
-  auto fields_at = [&](uint32_t g, T* dst) {
-    const T vx = UF(VX, g), vy = UF(VY, g), vz = UF(VZ, g);
-    const T b0x = B3(Ap->box, g), b0y = B3(Ap->boy, g), b0z = B3(Ap->boz, g);
-    dst[0] = vz;
-    dst[1] = UF(P, g);
-    dst[2] = UF(BX, g);
-    dst[3] = UF(BY, g);
-    dst[4] = -vx * b0z + vz * b0x;
-    dst[5] = -vz * b0y + vy * b0z;
-  };
-
+  // The window stops at plane nz-6 (planes nz-11 .. nz-1), which the
+  // one-sided rows of the top five planes read; a chunk that starts above
+  // nz-6 fills it there.
   T q[6][11], nq[6];
+  const int kq = min(k0, nz - 6);
 #pragma unroll
   for (int m = 0; m < 10; ++m) {
     T t[6];
-    fetch(k0 - 5 + m, t);
+    fetch(kq - 5 + m, t);
 #pragma unroll
     for (int f = 0; f < 6; ++f) q[f][m] = t[f];
   }
-  fetch(k0 + 5, nq);
+  fetch(kq + 5, nq);
 
   for (int k = k0; k < k1; ++k) {
     SPARC_REFRESH(Ap);
+    if (k <= nz - 6 || k == k0) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f) q[f][10] = nq[f];
-    fetch(k + 6, nq);
+      for (int f = 0; f < 6; ++f) q[f][10] = nq[f];
+    }
+    if (k < nz - 6) fetch(k + 6, nq);
     const T st = ((CPtr<T>)Ap->stretch)[k];
     T stl = 0;
     if constexpr (sizeof(T) == 4) stl = ((CPtr<T>)Ap->stretch_lo)[k];
@@ -873,18 +866,16 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
         d[f] = scl(st, stl, acc);
       }
     } else {
-      // top block: mirrored one-sided rows over planes nz-1 .. nz-11
+      // top block: mirrored one-sided rows over planes nz-1 .. nz-11, held
+      // in the window as q[f][10] .. q[f][0]
       const int r = nz - 1 - k;
-      T acc[6] = {0, 0, 0, 0, 0, 0};
-      for (int m = 0; m < 11; ++m) {
-        T v[6];
-        fields_at(ij + (uint32_t)(nz - 1 - m) * nxy, v);
-        const T c = Coef<T>::b(r, m);
 #pragma unroll
-        for (int f = 0; f < 6; ++f) acc[f] += v[f] * c;
+      for (int f = 0; f < 6; ++f) {
+        T acc = 0;
+#pragma unroll
+        for (int m = 0; m < 11; ++m) acc += q[f][10 - m] * Coef<T>::b(r, m);
+        d[f] = -scl(st, stl, acc);
       }
-#pragma unroll
-      for (int f = 0; f < 6; ++f) d[f] = -scl(st, stl, acc[f]);
     }
     if (active) {
       const uint32_t g = ij + (uint32_t)k * nxy;
@@ -900,10 +891,12 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
 #pragma unroll
       for (int f = 0; f < 6; ++f) stv<kStream>(Buf<T>(Ap->dz[f], fs), g, d[f]);
     }
+    if (k < nz - 6) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f)
+      for (int f = 0; f < 6; ++f)
 #pragma unroll
-      for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
+        for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
+    }
   }
 }
 
