@@ -307,10 +307,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   // field.  Quiet: vz, p.  MHD: vz, p, bx, by, flux1, flux2.  (Unused with ZP.)
   V q[NQ][11];
   V nq[NQ];  // next plane, loaded one iteration ahead
-  // planes outside 0..nz-1 read as 0: the zero ghosts below the bottom
+  // Planes below 0 read as 0: the zero ghosts below the bottom.  A plane
+  // kk >= nz, which no centred row reads, is loaded as plane kk-11: from
+  // plane nz-5 up the window then holds exactly planes nz-11 .. nz-1, the
+  // one-sided rows of the top five planes, so those come from registers
+  // too.  (Loading them per top plane costs 11 dependent round trips per
+  // plane, which set the length of a launch whose top chunk finishes last.)
   auto fetch = [&](int kk, V* dst) {
     if constexpr (ZP) return;
-    const uint32_t bo = (active && kk >= 0 && kk < nz) ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
+    kk = kk >= nz ? kk - 11 : kk;
+    const uint32_t bo = (active && kk >= 0) ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
     dst[0] = UFb(VZ, bo);
     dst[1] = UFb(P, bo);
     if constexpr (MHD) {
@@ -321,22 +327,15 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       dst[5] = -vx * B3b(Ap->boz, bo) + vz * B3b(Ap->box, bo);
     }
   };
-  // The window stops moving at plane nz-6, where it holds planes nz-11 ..
-  // nz-1: the one-sided rows of the top five planes read exactly those, so
-  // they come from registers too.  A chunk that starts above nz-6 fills the
-  // window at that position.  (Loading the 11 rows per top plane from memory
-  // instead costs 11 dependent round trips per plane, which set the length
-  // of a whole launch when the top chunk is the last block to finish.)
-  const int kq = min(k0, nz - 6);
   if constexpr (!ZP) {
 #pragma unroll
     for (int m = 0; m < 10; ++m) {
       V t[NQ];
-      fetch(kq - 5 + m, t);
+      fetch(k0 - 5 + m, t);
 #pragma unroll
       for (int f = 0; f < NQ; ++f) q[f][m] = t[f];
     }
-    fetch(kq + 5, nq);
+    fetch(k0 + 5, nq);
   }
 
   double ke = 0.0, mr = 1.0e300, mp = 1.0e300;
@@ -428,6 +427,14 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     SPARC_REFRESH(Ap);
     L = LB[NB == 2 ? (k & 1) : 0];
     if constexpr (SPF) {
+      // Every load of the previous iteration (this plane's staging and point
+      // values, the window's next plane) completes here, in one wait, before
+      // the next plane's loads issue.  Left to itself the compiler may defer
+      // part of it to the first use; the in-order load counter then makes
+      // that later wait cover the next plane's loads too, and the prefetch
+      // stops overlapping the compute (quiet 512x512x300 FP64 stage 2.21 ->
+      // 2.63 ms).  0x0F70: vmcnt 0, expcnt and lgkmcnt not waited on.
+      __builtin_amdgcn_s_waitcnt(0x0F70);
       pre_store();
       if constexpr (PPF) {
 #pragma unroll
@@ -447,11 +454,9 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     // ---- z derivatives (derivatives.f90:481-487,509-518,536-545) ----
     V dz[NQ];
     if constexpr (!ZP) {
-      if (k <= nz - 6 || k == k0) {
 #pragma unroll
-        for (int f = 0; f < NQ; ++f) q[f][10] = nq[f];
-      }
-      if (k < nz - 6) fetch(k + 6, nq);
+      for (int f = 0; f < NQ; ++f) q[f][10] = nq[f];
+      fetch(k + 6, nq);
       if (active) {
         const T st = ((CPtr<T>)Ap->stretch)[k];
         T stl = 0;
@@ -471,13 +476,14 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           }
         } else {
           // top block: mirrored one-sided rows over planes nz-1 .. nz-11,
-          // held in the window as q[f][10] .. q[f][0]
+          // all in the window (see fetch): row m of plane nz-1-r sits in
+          // slot (r + 16 - m) mod 11, so the sum runs over the slots
           const int r = nz - 1 - k;
 #pragma unroll
           for (int f = 0; f < NQ; ++f) {
             V acc = 0;
 #pragma unroll
-            for (int m = 0; m < 11; ++m) acc += q[f][10 - m] * Coef<T>::b(r, m);
+            for (int m = 0; m < 11; ++m) acc += q[f][m] * Coef<T>::b(r, (r + 16 - m) % 11);
             dz[f] = -scl(st, stl, acc);
           }
         }
@@ -493,7 +499,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       const int cc = (ty + 5) * LX + tx + HX;
       const V vx = L[0][cc], vy = L[1][cc];
       const V rho = PPF ? pc[0] : (MPX ? pm[0] : UF(RHO, g));
-      const V vz = ZP ? (MPX ? pm[1] : UF(VZ, g)) : (k <= nz - 6 ? q[0][5] : UF(VZ, g)), p = L[2][cc];
+      const V vz = ZP ? (MPX ? pm[1] : UF(VZ, g)) : q[0][5], p = ZP ? L[2][cc] : q[1][5];
       // z pass output, in CPML family order: vz, p, bx, by, F2, F1
       auto DZ = [&](int f) -> V { return MPX ? pm[2 + f] : ldv<V>(Buf<T>(Ap->dz[f], fs), g * W); };
       V dvzdz = ZP ? DZ(0) : dz[0], gpz = ZP ? DZ(1) : dz[1];
@@ -641,7 +647,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         const V red = MPX ? PMB(3) : B3(Ap->red, g);
         const V b0x = MPX ? PMB(4) : B3(Ap->box, g), b0y = MPX ? PMB(5) : B3(Ap->boy, g),
                 b0z = MPX ? PMB(6) : B3(Ap->boz, g);
-        const V bx = L[8][cc], by = L[6][cc];
+        const V bx = ZP ? L[8][cc] : q[2][5], by = ZP ? L[6][cc] : q[3][5];
         // RHS.f90:548-580
         K[BX] = red * (dyf3 - dzf2);
         K[BY] = red * (dzf1 - dxf3);
@@ -742,12 +748,10 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     if constexpr (NB == 1) __syncthreads();
 
     if constexpr (!ZP) {
-      if (k < nz - 6) {
 #pragma unroll
-        for (int f = 0; f < NQ; ++f)
+      for (int f = 0; f < NQ; ++f)
 #pragma unroll
-          for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
-      }
+        for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
     }
   }
 
@@ -814,10 +818,13 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
   // once: all non-temporal (256^3 MHD z pass 0.481 -> 0.477 ms FP64)
   auto UFs = [&](int f, uint32_t bo) -> T { return ldv<T, kStream>(Buf<T>(Ap->u[f], fs), bo); };
   auto B3s = [&](const T* p, uint32_t bo) -> T { return ldv<T, kStream>(Buf<T>(p, fs), bo); };
-  // the six fields at plane kk in family order; outside 0..nz-1 they read 0
-  // (the zero ghosts below the bottom)
+  // the six fields at plane kk in family order; below plane 0 they read 0
+  // (the zero ghosts below the bottom); a plane kk >= nz, which no centred
+  // row reads, is loaded as plane kk-11, so from plane nz-5 up the window
+  // holds planes nz-11 .. nz-1, the top five planes' one-sided rows
   auto fetch = [&](int kk, T* dst) {
-    const uint32_t bo = (active && kk >= 0 && kk < nz) ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
+    kk = kk >= nz ? kk - 11 : kk;
+    const uint32_t bo = (active && kk >= 0) ? (ij + (uint32_t)kk * nxy) * W : Buf<T>::kOut;
     const T vx = UFs(VX, bo), vy = UFs(VY, bo), vz = UFs(VZ, bo);
     const T b0x = B3s(Ap->box, bo), b0y = B3s(Ap->boy, bo), b0z = B3s(Ap->boz, bo);
     dst[0] = vz;
@@ -827,27 +834,21 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
     dst[4] = -vx * b0z + vz * b0x;  // F2
     dst[5] = -vz * b0y + vy * b0z;  // F1
   };
-  // The window stops at plane nz-6 (planes nz-11 .. nz-1), which the
-  // one-sided rows of the top five planes read; a chunk that starts above
-  // nz-6 fills it there.
   T q[6][11], nq[6];
-  const int kq = min(k0, nz - 6);
 #pragma unroll
   for (int m = 0; m < 10; ++m) {
     T t[6];
-    fetch(kq - 5 + m, t);
+    fetch(k0 - 5 + m, t);
 #pragma unroll
     for (int f = 0; f < 6; ++f) q[f][m] = t[f];
   }
-  fetch(kq + 5, nq);
+  fetch(k0 + 5, nq);
 
   for (int k = k0; k < k1; ++k) {
     SPARC_REFRESH(Ap);
-    if (k <= nz - 6 || k == k0) {
 #pragma unroll
-      for (int f = 0; f < 6; ++f) q[f][10] = nq[f];
-    }
-    if (k < nz - 6) fetch(k + 6, nq);
+    for (int f = 0; f < 6; ++f) q[f][10] = nq[f];
+    fetch(k + 6, nq);
     const T st = ((CPtr<T>)Ap->stretch)[k];
     T stl = 0;
     if constexpr (sizeof(T) == 4) stl = ((CPtr<T>)Ap->stretch_lo)[k];
@@ -866,14 +867,15 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
         d[f] = scl(st, stl, acc);
       }
     } else {
-      // top block: mirrored one-sided rows over planes nz-1 .. nz-11, held
-      // in the window as q[f][10] .. q[f][0]
+      // top block: mirrored one-sided rows over planes nz-1 .. nz-11, all
+      // in the window (see fetch): row m of plane nz-1-r sits in slot
+      // (r + 16 - m) mod 11
       const int r = nz - 1 - k;
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
         T acc = 0;
 #pragma unroll
-        for (int m = 0; m < 11; ++m) acc += q[f][10 - m] * Coef<T>::b(r, m);
+        for (int m = 0; m < 11; ++m) acc += q[f][m] * Coef<T>::b(r, (r + 16 - m) % 11);
         d[f] = -scl(st, stl, acc);
       }
     }
@@ -891,12 +893,10 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
 #pragma unroll
       for (int f = 0; f < 6; ++f) stv<kStream>(Buf<T>(Ap->dz[f], fs), g, d[f]);
     }
-    if (k < nz - 6) {
 #pragma unroll
-      for (int f = 0; f < 6; ++f)
+    for (int f = 0; f < 6; ++f)
 #pragma unroll
-        for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
-    }
+      for (int m = 0; m < 10; ++m) q[f][m] = q[f][m + 1];
   }
 }
 
