@@ -52,7 +52,6 @@ template <> struct Coef<double> {
     constexpr double v[5] = SPARC_DPC;
     return v[m];
   }
-  __device__ static double b(int r, int m) { return DPB_D[r][m]; }
 };
 template <> struct Coef<float> {
   __device__ static float c(int m) {
@@ -64,7 +63,6 @@ template <> struct Coef<float> {
     constexpr double v[5] = SPARC_DPC;
     return (float)(v[m] - (double)(float)v[m]);
   }
-  __device__ static float b(int r, int m) { return (float)DPB_D[r][m]; }
 };
 
 // FP32 keeps each uniform scale of the scheme -- the derivative metric
@@ -271,6 +269,47 @@ __device__ __forceinline__ F2& operator+=(F2& a, F2 b) { return a = a + b; }
 // V = the value one lane holds: T (one point) or F2 (two points, FP32 only)
 template <typename T, int VW> struct Lane { using V = T; };
 template <> struct Lane<float, 2> { using V = F2; };
+
+// One-sided boundary row sum_m b_m v_m, with the row's FP64 coefficients b_m
+// (derivatives.f90:20-30: the five rows next to a non-periodic x or y face
+// and below the top in z).  FP64: multiply-adds in row order.  FP32: these
+// coefficients reach 7.9 in magnitude and sum to zero, so a plain sum's
+// rounding error is ~30 ulp of the values, and over thousands of steps it,
+// not the centred rows, sets the FP32 error next to those faces (where the
+// flux-tube runs carry their largest velocities).  The sum is therefore
+// compensated -- Ogita, Rump & Oishi (2005), Dot2: each product split
+// exactly with an FMA, the coefficient's FP64 rest taken as one more
+// product, each addition split exactly with TwoSum -- which is as accurate
+// as the plain sum in twice the working precision, rounded once.
+template <typename T> struct RowSum {
+  T p = 0, s = 0;
+  __device__ __forceinline__ void add(T v, double b) {
+    if constexpr (sizeof(T) == 8) {
+      p += v * b;
+    } else {
+      const float bh = (float)b, bl = (float)(b - (double)bh);
+      const float h = bh * v;
+      const float e = __builtin_fmaf(bh, v, -h) + bl * v;
+      const float t = p + h, z = t - p;
+      s += ((p - (t - z)) + (h - z)) + e;
+      p = t;
+    }
+  }
+  __device__ __forceinline__ T sum() const {
+    if constexpr (sizeof(T) == 8) return p;
+    else return p + s;
+  }
+};
+struct RowSumF2 {
+  RowSum<float> a, b;
+  __device__ __forceinline__ void add(F2 v, double c) {
+    a.add(v.x, c);
+    b.add(v.y, c);
+  }
+  __device__ __forceinline__ F2 sum() const { return F2(a.sum(), b.sum()); }
+};
+template <typename V> struct RowSumOf { using type = RowSum<V>; };
+template <> struct RowSumOf<F2> { using type = RowSumF2; };
 
 // point c of a lane value
 template <typename T> __device__ __forceinline__ T lane_pt(T v, int) { return v; }

@@ -261,17 +261,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
 #pragma unroll
       for (int c = 0; c < VW; ++c) {
         const int p = i + c;
-        T acc = 0;
+        RowSum<T> acc;
         if (p < 5) {
 #pragma unroll
-          for (int m = 0; m < 11; ++m) acc += rp[min(m - i0 + XO, RW - 1)] * Coef<T>::b(p, m);
-          set_lane_pt(out, c, scl(sxs, sxl, acc));
+          for (int m = 0; m < 11; ++m) acc.add(rp[min(m - i0 + XO, RW - 1)], DPB_D[p][m]);
+          set_lane_pt(out, c, scl(sxs, sxl, acc.sum()));
         } else if (p >= nx - 5) {
           const int r = nx - 1 - p;
 #pragma unroll
-          for (int m = 0; m < 11; ++m)
-            acc += rp[min(max(nx - 1 - m - i0 + XO, 0), RW - 1)] * Coef<T>::b(r, m);
-          set_lane_pt(out, c, -scl(sxs, sxl, acc));
+          for (int m = 0; m < 11; ++m) acc.add(rp[min(max(nx - 1 - m - i0 + XO, 0), RW - 1)], DPB_D[r][m]);
+          set_lane_pt(out, c, -scl(sxs, sxl, acc.sum()));
         }
       }
     }
@@ -279,18 +278,20 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   };
   auto dy_lds = [&](int f) -> V {
     const V* col = &L[f][tx + HX];
-    V acc = 0;
     if (!periodic && j < 5) {
+      typename RowSumOf<V>::type e;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) acc += col[min(max(m - j0 + 5, 0), LY - 1) * LX] * Coef<T>::b(j, m);
-      return scl(sys, syl, acc);
+      for (int m = 0; m < 11; ++m) e.add(col[min(max(m - j0 + 5, 0), LY - 1) * LX], DPB_D[j][m]);
+      return scl(sys, syl, e.sum());
     }
     if (!periodic && j >= ny - 5) {
       const int r = ny - 1 - j;
+      typename RowSumOf<V>::type e;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) acc += col[min(max(ny - 1 - m - j0 + 5, 0), LY - 1) * LX] * Coef<T>::b(r, m);
-      return -scl(sys, syl, acc);
+      for (int m = 0; m < 11; ++m) e.add(col[min(max(ny - 1 - m - j0 + 5, 0), LY - 1) * LX], DPB_D[r][m]);
+      return -scl(sys, syl, e.sum());
     }
+    V acc = 0;
     const int r0 = ty + 5;
     V acl = 0;
 #pragma unroll
@@ -481,10 +482,10 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           const int r = nz - 1 - k;
 #pragma unroll
           for (int f = 0; f < NQ; ++f) {
-            V acc = 0;
+            typename RowSumOf<V>::type acc;
 #pragma unroll
-            for (int m = 0; m < 11; ++m) acc += q[f][m] * Coef<T>::b(r, (r + 16 - m) % 11);
-            dz[f] = -scl(st, stl, acc);
+            for (int m = 0; m < 11; ++m) acc.add(q[f][m], DPB_D[r][(r + 16 - m) % 11]);
+            dz[f] = -scl(st, stl, acc.sum());
           }
         }
       }
@@ -873,10 +874,10 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
       const int r = nz - 1 - k;
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
-        T acc = 0;
+        RowSum<T> acc;
 #pragma unroll
-        for (int m = 0; m < 11; ++m) acc += q[f][m] * Coef<T>::b(r, (r + 16 - m) % 11);
-        d[f] = -scl(st, stl, acc);
+        for (int m = 0; m < 11; ++m) acc.add(q[f][m], DPB_D[r][(r + 16 - m) % 11]);
+        d[f] = -scl(st, stl, acc.sum());
       }
     }
     if (active) {

@@ -12,8 +12,9 @@
 * quiet_precision (512 x 512 x 300): FP32 + compensated update against
   FP64 after 500 steps, relative L2 <= 1e-5 per field.
 * batch_population (the 64 seeded runs): batched equals alone, bit for bit,
-  on a sample of the runs; one run of it in FP32 stays within 1e-5 of its
-  FP64 twin after 2000 steps.
+  on a sample of the runs; the three smallest runs that stay within the
+  failure checks, batched in FP32, stay within 1e-5 of their FP64 twins
+  after 2000 steps.
 * a Poisson request stream with the served form's grids (64^3 to
   384 x 384 x 256) at reduced step counts: each result equals the request
   served alone.
@@ -198,20 +199,41 @@ def test_batch_population_equals_alone():
 
 
 def test_batch_population_run_fp32_after_2000_steps():
-    """The three smallest population runs in FP32, batched, against the
-    same runs alone in FP64 (which equal the oracle to 1e-11) after 2000
-    steps: relative L2 <= 1e-5 per field."""
+    """The three smallest population runs that stay within the reference's
+    failure checks for 2000 FP64 steps, batched in FP32, against the same
+    runs alone in FP64 (which equal the oracle to 1e-11) after 2000 steps:
+    relative L2 <= 1e-5 per field.  Several small flux-tube runs grow until
+    p + p0 turns negative after 1400-1850 s of simulated time whatever the
+    time step (the linearised equations about these generated backgrounds
+    are unstable; the oracle grows alike), so runs are taken in size order
+    and those whose FP64 run trips the checks are passed over."""
+    from sparc2_amd._lib import GUARD_RECORD
     from sparc2_amd.batch import Batch
     from sparc2_amd.solver import Solver
     pop = _population()
-    pick = sorted(range(64), key=lambda q: pop[q][0].npoints)[:3]
-    sol = [Solver(pop[q][0], pop[q][1]) for q in pick]
-    b = Batch(sol, use_graph=True)
-    b.advance(2000)
-    for q, s32 in zip(pick, sol):
+    pick, ref = [], {}
+    for q in sorted(range(64), key=lambda q: pop[q][0].npoints):
         c, bgq = pop[q]
-        s64 = Solver(c.replace(precision="fp64"), bgq, setup=s32.st)
-        s64.advance(0, 1999)
+        s64 = Solver(c.replace(precision="fp64"), bgq)
+        s64.set_guard(GUARD_RECORD)          # note the first failure, keep stepping
+        s64.advance(0, 1999, check_guard=False)
+        failed, t, _, _ = s64.guard()
+        _say(f"run {q} ({c.nx}x{c.ny}x{c.nz}) FP64 2000 steps: " +
+             (f"fails the checks at step {t}, passed over" if failed else "within the checks"))
+        if failed:
+            s64.close()
+            continue
+        pick.append(q)
+        ref[q] = s64
+        if len(pick) == 3:
+            break
+    assert len(pick) == 3
+    sol = [Solver(pop[q][0], pop[q][1], setup=ref[q].st) for q in pick]
+    b = Batch(sol, use_graph=True)
+    assert not b.advance(2000)
+    for q, s32 in zip(pick, sol):
+        c = pop[q][0]
+        s64 = ref[q]
         errs = field_errors(s32.get_state(), s64.get_state())
         _say(f"run {q} ({c.nx}x{c.ny}x{c.nz}, {'MHD' if c.mhd else 'quiet'}) FP32 batched vs FP64 "
              f"after 2000 steps: " + " ".join(f"{e:.1e}" for e in errs))
