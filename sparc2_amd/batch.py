@@ -52,13 +52,15 @@ def _apply(members: Sequence[Solver], tile, auto, mult):
         s.set_tile((tile[0], tile[1], 0 if mult == 1 else min(k * mult, s.cfg.nz)))
 
 
-# Tuned choice per variant group, keyed by the group's variant and the
-# sorted tile keys of its runs: a group that comes back (the served form
-# rebuilds its batch whenever runs join or leave) is not timed again.
+# Tuned choices: keyed by the group's variant and the sorted tile keys of
+# its runs (a group that comes back is not timed again), or with by_variant
+# by the variant, precision and lane setting alone -- the served form, whose
+# membership changes at every arrival and departure, times a variant once
+# and reuses its shape for every later group of that variant that it fits.
 _GROUP_CACHE: Dict[tuple, tuple] = {}
 
 
-def tune_tiles(solvers: Sequence[Solver], reps: int = 3) -> Dict[tuple, tuple]:
+def tune_tiles(solvers: Sequence[Solver], reps: int = 3, by_variant: bool = False) -> Dict[tuple, tuple]:
     """Tile autotuning for a batch.  Runs that share a kernel variant share
     one launch, so the shape is chosen per variant group, by timing a middle
     RK stage of that group's batched launch for every shape that fits all of
@@ -71,15 +73,23 @@ def tune_tiles(solvers: Sequence[Solver], reps: int = 3) -> Dict[tuple, tuple]:
         groups.setdefault((s.cfg.magnetic == 2, bool(s.st.bg3d)), []).append(s)
     chosen = {}
     for key, members in groups.items():
-        ckey = (key, tuple(sorted(s.tile_key() for s in members)))
+        if by_variant:
+            tk = members[0].tile_key()
+            ckey = ("variant", key, tk[4], tk[-1])
+        else:
+            ckey = (key, tuple(sorted(s.tile_key() for s in members)))
         if ckey in _GROUP_CACHE:
             t, mult = _GROUP_CACHE[ckey]
-            for s in members:
-                s.set_tile((t[0], t[1], 0))
-            auto = [s.tile()[0][2] for s in members]
-            _apply(members, t, auto, mult)
-            chosen[key] = (t[0], t[1], mult)
-            continue
+            try:
+                for s in members:
+                    s.set_tile((t[0], t[1], 0))
+            except SparcError:
+                pass                    # the shape does not fit a new grid: time again
+            else:
+                auto = [s.tile()[0][2] for s in members]
+                _apply(members, t, auto, mult)
+                chosen[key] = (t[0], t[1], mult)
+                continue
         best, best_ms, best_auto = None, float("inf"), None
         for t in TILES:
             try:
@@ -108,11 +118,12 @@ def tune_tiles(solvers: Sequence[Solver], reps: int = 3) -> Dict[tuple, tuple]:
 
 
 class Batch:
-    def __init__(self, solvers: Sequence[Solver], use_graph: bool = True, autotune: bool = False):
+    def __init__(self, solvers: Sequence[Solver], use_graph: bool = True, autotune: bool = False,
+                 tune_by_variant: bool = False):
         self.solvers: List[Solver] = list(solvers)
         if not self.solvers:
             raise ValueError("empty batch")
-        self.tiles = tune_tiles(self.solvers) if autotune else None
+        self.tiles = tune_tiles(self.solvers, by_variant=tune_by_variant) if autotune else None
         self.h = _create(self.solvers, use_graph)
         self.time = [0] * len(self.solvers)      # next time index of each run
         self._deferred = None

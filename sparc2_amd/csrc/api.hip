@@ -32,7 +32,11 @@ int zpass_batch_blocks(bool f32, int ncu);
 bool stage_tile_supported(int tx, int ty);
 template <typename T>
 hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
-                         int nfield, int periodic, T s, int bare, hipStream_t st, const int* halt);
+                         int nfield, int periodic, T s, int bare, hipStream_t st, const int* halt,
+                         bool inplace = false);
+template <typename T>
+hipError_t launch_filter_xy(int g, const T* in, T* out, int nx, int ny, int nz, long fs, int nfield,
+                            int periodic, T s, hipStream_t st, const int* halt);
 hipError_t launch_deriv(int axis, const double* in, double* out, int nx, int ny, int nz, double sxy,
                         const double* stretch, int bc, hipStream_t st);
 hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, const GuardArgs& g,
@@ -828,18 +832,25 @@ int sparc_hip_step(sparc_ctx* c, double amp) {
   return c->f32 ? do_step<float>(c, amp) : do_step<double>(c, amp);
 }
 
+// One filter pass out of place, then the buffer roles swap.  axis 2: z;
+// axis -1: the fused y-then-x pass.
 template <typename T>
 static int filter_pass(sparc_ctx* c, int axis, int g, T s) {
+  auto launch = [&](const void* in, void* out) -> hipError_t {
+    if (axis < 0)
+      return launch_filter_xy<T>(g, (const T*)in, (T*)out, c->nx, c->ny, c->nz, c->fs, c->nv,
+                                 c->cfg.periodic, s, c->st, c->guard);
+    return launch_filter<T>(axis, g, (const T*)in, (T*)out, c->nx, c->ny, c->nz, c->fs, c->nv,
+                            c->cfg.periodic, s, 0, c->st, c->guard);
+  };
   TimeRec r;
   timed_begin(c, 3, &r);
-  hipError_t e = launch_filter<T>(axis, g, (const T*)c->pool[c->ra], (T*)c->pool[c->rk0], c->nx,
-                                  c->ny, c->nz, c->fs, c->nv, c->cfg.periodic, s, 0, c->st, c->guard);
+  hipError_t e = launch(c->pool[c->ra], c->pool[c->rk0]);
   timed_end(c, &r);
   if (e != hipSuccess) return fail(SPARC_E_HIP, "filter launch failed");
   std::swap(c->ra, c->rk0);
   if (c->f32) {  // the compensation term is filtered with the same linear operator
-    e = launch_filter<T>(axis, g, (const T*)c->comp[c->rc], (T*)c->comp[1 - c->rc], c->nx, c->ny,
-                         c->nz, c->fs, c->nv, c->cfg.periodic, s, 0, c->st, c->guard);
+    e = launch(c->comp[c->rc], c->comp[1 - c->rc]);
     if (e != hipSuccess) return fail(SPARC_E_HIP, "filter launch failed");
     c->rc = 1 - c->rc;
   }
@@ -850,8 +861,7 @@ template <typename T>
 static int do_filter(sparc_ctx* c, int time) {
   int rc;
   if (time % c->cfg.xyfreq == 0) {
-    if ((rc = filter_pass<T>(c, 1, c->cfg.g_xy_filt, (T)c->cfg.s_xy))) return rc;
-    if ((rc = filter_pass<T>(c, 0, c->cfg.g_xy_filt, (T)c->cfg.s_xy))) return rc;
+    if ((rc = filter_pass<T>(c, -1, c->cfg.g_xy_filt, (T)c->cfg.s_xy))) return rc;
   }
   if (time % c->cfg.zfreq == 0) {
     if ((rc = filter_pass<T>(c, 2, c->cfg.g_z_filt, (T)c->cfg.s_z))) return rc;
@@ -1240,37 +1250,29 @@ static int batch_enqueue_step(sparc_batch* b) {
   return SPARC_OK;
 }
 
-// FILTER_VARS for one run on the batch stream, written back into the run's
-// own buffers (the device stage arguments keep pointing at them).
+// FILTER_VARS for one run on the batch stream, ending in the run's own
+// buffers (the device stage arguments keep pointing at them): the fused xy
+// pass into the spare buffer, then the z pass back out of place when both
+// cadences meet; the xy pass alone is copied back; the z pass alone runs in
+// place (one chunk per column).
 template <typename T>
 static int batch_filter(sparc_batch* b, sparc_ctx* c, int time) {
-  auto pass = [&](int axis, int g, T sv, const void* in, void* out, bool comp) -> hipError_t {
-    (void)comp;
-    return launch_filter<T>(axis, g, (const T*)in, (T*)out, c->nx, c->ny, c->nz, c->fs, c->nv,
-                            c->cfg.periodic, sv, 0, b->st, c->guard);
-  };
-  void* a = c->pool[c->ra];
-  void* t = c->pool[c->rk0];
   const long n = (long)c->nv * c->fs;
-  hipError_t e = hipSuccess;
-  if (time % c->cfg.xyfreq == 0) {
-    const T sv = (T)c->cfg.s_xy;
-    e = pass(1, c->cfg.g_xy_filt, sv, a, t, false);
-    if (e == hipSuccess) e = pass(0, c->cfg.g_xy_filt, sv, t, a, false);
-    if (e == hipSuccess && c->f32) {
-      e = pass(1, c->cfg.g_xy_filt, sv, c->comp[c->rc], c->comp[1 - c->rc], true);
-      if (e == hipSuccess) e = pass(0, c->cfg.g_xy_filt, sv, c->comp[1 - c->rc], c->comp[c->rc], true);
-    }
-  }
-  if (e == hipSuccess && time % c->cfg.zfreq == 0) {
-    const T sv = (T)c->cfg.s_z;
-    e = pass(2, c->cfg.g_z_filt, sv, a, t, false);
-    if (e == hipSuccess) e = launch_copy(t, a, n, c->f32, b->st, c->guard);
-    if (e == hipSuccess && c->f32) {
-      e = pass(2, c->cfg.g_z_filt, sv, c->comp[c->rc], c->comp[1 - c->rc], true);
-      if (e == hipSuccess) e = launch_copy(c->comp[1 - c->rc], c->comp[c->rc], n, true, b->st, c->guard);
-    }
-  }
+  const bool xy = time % c->cfg.xyfreq == 0, z = time % c->cfg.zfreq == 0;
+  auto run = [&](void* a, void* t) -> hipError_t {
+    hipError_t e = hipSuccess;
+    if (xy)
+      e = launch_filter_xy<T>(c->cfg.g_xy_filt, (const T*)a, (T*)t, c->nx, c->ny, c->nz, c->fs, c->nv,
+                              c->cfg.periodic, (T)c->cfg.s_xy, b->st, c->guard);
+    if (e == hipSuccess && z)
+      e = launch_filter<T>(2, c->cfg.g_z_filt, (const T*)(xy ? t : a), (T*)a, c->nx, c->ny, c->nz,
+                           c->fs, c->nv, c->cfg.periodic, (T)c->cfg.s_z, 0, b->st, c->guard, !xy);
+    else if (e == hipSuccess && xy)
+      e = launch_copy(t, a, n, c->f32, b->st, c->guard);
+    return e;
+  };
+  hipError_t e = run(c->pool[c->ra], c->pool[c->rk0]);
+  if (e == hipSuccess && c->f32) e = run(c->comp[c->rc], c->comp[1 - c->rc]);
   if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch filter: ") + hipGetErrorString(e));
   return SPARC_OK;
 }

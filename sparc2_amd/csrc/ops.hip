@@ -90,6 +90,109 @@ __global__ __launch_bounds__(256) void k_filter(const T* __restrict__ in, T* __r
   out[(long)f * fs + idx] = (G == 5 || bare) ? filt : sm(pos) - s * filt;
 }
 
+// FILTER_VARS's y and x passes in one kernel (derivatives.f90:669-687):
+// y filter, then x filter of its result, each with its edge smoothing, one
+// read and one write per value.  A wave owns 64 consecutive x points (one per
+// lane, so loads and stores are coalesced) of one row segment and marches up
+// a chunk of KY rows with a register queue of the raw rows j-H .. j+H, loaded
+// PD rows ahead: the y stencil is read from registers.  The x stencil of the
+// y-filtered row is read from the neighbouring lanes (wavefront shuffles), so
+// the inner 64 - 2H lanes write results; the outer H lanes on each side only
+// supply the stencil (H = G + 1: the filter's reach plus the one-point reach
+// of the smoothing).  The per-point arithmetic is that of the separate passes,
+// in the same order, so both give the same bits.
+constexpr int FXY_KY = 64;
+constexpr int FXY_PD = 4;
+
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_filter_xy(const T* __restrict__ in, T* __restrict__ out,
+                                                   int nx, int ny, int nz, long fs, int nseg, int nych,
+                                                   long nwave, int periodic, T s,
+                                                   const int* __restrict__ halt) {
+  constexpr int H = G + 1, OW = 64 - 2 * H, Q = 2 * H + 1;
+  const int nblk = gridDim.x, b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
+  const long lg = xcd * base + (xcd < rem ? xcd : rem) + loc;
+  const long wv = lg * 4 + (threadIdx.x >> 6);
+  if (wv >= nwave) return;  // whole waves only: the shuffles below see full waves
+  const int lane = threadIdx.x & 63;
+  const int seg = (int)(wv % nseg);
+  long rest = wv / nseg;
+  const int ych = (int)(rest % nych);
+  rest /= nych;
+  const int k = (int)(rest % nz);
+  const int f = (int)(rest / nz);
+  const int x0 = seg * OW;
+  const int xl = x0 - H + lane;  // this lane's x (unwrapped)
+  const bool xin = periodic || (xl >= 0 && xl < nx);
+  const int xg = periodic ? wrap(xl, nx) : xl;
+  const bool writer = lane >= H && lane < 64 - H && xl < nx;
+  const int j0 = ych * FXY_KY, j1 = min(j0 + FXY_KY, ny);
+  const long plane = (long)f * fs + (long)nx * ny * k;
+  const T* col = in + plane + (xin ? xg : 0);
+  T* ocol = out + plane + xl;
+  if (halted(halt)) {  // a run stopped by its failure guard keeps its state
+    if (writer)
+      for (int j = j0; j < j1; ++j) ocol[(long)j * nx] = col[(long)j * nx];
+    return;
+  }
+  auto ld = [&](int r) -> T {
+    if (periodic) return col[(long)wrap(r, ny) * nx];
+    return (xin && r >= 0 && r < ny) ? col[(long)r * nx] : T(0);
+  };
+  T q[Q], pf[FXY_PD];
+#pragma unroll
+  for (int m = 0; m < Q - 1; ++m) q[m] = ld(j0 - H + m);
+#pragma unroll
+  for (int d = 0; d < FXY_PD; ++d) pf[d] = ld(j0 + H + d);
+  const bool xedge = !periodic && ((xl >= 1 && xl <= G - 1) || (xl >= nx - G && xl <= nx - 2));
+  const bool xcopy = !periodic && (xl < G || xl >= nx - G);
+  for (int j = j0; j < j1; ++j) {
+    q[Q - 1] = pf[0];
+#pragma unroll
+    for (int d = 0; d < FXY_PD - 1; ++d) pf[d] = pf[d + 1];
+    pf[FXY_PD - 1] = ld(j + H + FXY_PD);
+    // y pass: smoothed raw value of row j+d (derivatives.f90:671-675)
+    auto smy = [&](int d) -> T {
+      const int r = j + d;
+      if (!periodic && ((r >= 1 && r <= G - 1) || (r >= ny - G && r <= ny - 2)))
+        return (q[d + H - 1] + q[d + H + 1]) / T(2);
+      return q[d + H];
+    };
+    T yf, y;
+    if (periodic) {
+      T acc = 0;
+#pragma unroll
+      for (int m = -G; m <= G; ++m) acc += q[m + H] * filt_w<T>(G, m);
+      yf = acc;
+      y = G == 5 ? yf : q[H] - s * yf;
+    } else if (j < G || j >= ny - G) {
+      yf = G == 5 ? smy(0) : T(0);  // copy (g = 5) or zero (g = 3)
+      y = G == 5 ? yf : smy(0) - s * yf;
+    } else {
+      T acc = 0;
+#pragma unroll
+      for (int m = -G; m <= G; ++m) acc += smy(m) * filt_w<T>(G, m);
+      yf = acc;
+      y = G == 5 ? yf : smy(0) - s * yf;
+    }
+    // x pass on the y-filtered row across the lanes (derivatives.f90:680-687)
+    const T ym = __shfl_up(y, 1), yp = __shfl_down(y, 1);
+    const T sx = xedge ? (ym + yp) / T(2) : y;
+    T acc = 0;
+#pragma unroll
+    for (int m = -G; m <= G; ++m) {
+      const T v = m < 0 ? __shfl_up(sx, -m) : (m > 0 ? __shfl_down(sx, m) : sx);
+      acc += v * filt_w<T>(G, m);
+    }
+    T xf = acc;
+    if (xcopy) xf = G == 5 ? sx : T(0);
+    if (writer) ocol[(long)j * nx] = G == 5 ? xf : sx - s * xf;
+#pragma unroll
+    for (int m = 0; m < Q - 1; ++m) q[m] = q[m + 1];
+  }
+}
+
 // z filter: one column per thread marching up a chunk of planes with a
 // register queue of the raw values (planes pos-G-1 .. pos+G+1: the filter
 // reads smoothed values at pos-G .. pos+G, and a smoothed value reads its two
@@ -98,7 +201,7 @@ __global__ __launch_bounds__(256) void k_filter(const T* __restrict__ in, T* __r
 constexpr int ZPD = 4;
 
 template <typename T, int G>
-__global__ __launch_bounds__(256) void k_filter_z(const T* __restrict__ in, T* __restrict__ out,
+__global__ __launch_bounds__(256) void k_filter_z(const T* in, T* out,
                                                   int nx, int ny, int nz, long fs, int ncb, int kc,
                                                   T s, int bare, const int* __restrict__ halt) {
   const bool stop = halted(halt);  // guard stop: copy, as in k_filter
@@ -151,14 +254,40 @@ __global__ __launch_bounds__(256) void k_filter_z(const T* __restrict__ in, T* _
 }
 
 template <typename T>
+hipError_t launch_filter_xy(int g, const T* in, T* out, int nx, int ny, int nz, long fs, int nfield,
+                            int periodic, T s, hipStream_t st, const int* halt) {
+  const int ow = 64 - 2 * (g + 1);
+  const int nseg = (nx + ow - 1) / ow, nych = (ny + FXY_KY - 1) / FXY_KY;
+  const long nwave = (long)nseg * nych * nz * nfield;
+  const long nb = (nwave + 3) / 4;
+  if (nb >= (1L << 31)) return hipErrorInvalidValue;
+  if (g == 5)
+    hipLaunchKernelGGL((k_filter_xy<T, 5>), dim3((unsigned)nb), dim3(256), 0, st, in, out, nx, ny, nz,
+                       fs, nseg, nych, nwave, periodic, s, halt);
+  else
+    hipLaunchKernelGGL((k_filter_xy<T, 3>), dim3((unsigned)nb), dim3(256), 0, st, in, out, nx, ny, nz,
+                       fs, nseg, nych, nwave, periodic, s, halt);
+  return hipGetLastError();
+}
+template hipError_t launch_filter_xy<double>(int, const double*, double*, int, int, int, long, int,
+                                             int, double, hipStream_t, const int*);
+template hipError_t launch_filter_xy<float>(int, const float*, float*, int, int, int, long, int, int,
+                                            float, hipStream_t, const int*);
+
+// bare: the filtx/filty/filtz operators alone.  inplace (z only): in may
+// equal out -- one chunk per column, whose register queue holds every raw
+// value it still needs before that plane is overwritten.
+template <typename T>
 hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
-                         int nfield, int periodic, T s, int bare, hipStream_t st, const int* halt) {
+                         int nfield, int periodic, T s, int bare, hipStream_t st, const int* halt,
+                         bool inplace) {
+  if (inplace && axis != 2) return hipErrorInvalidValue;
   if (axis == 2) {
     // column blocks x z chunks x fields: >= ~2048 blocks, chunks >= 16 deep
     const long nxy = (long)nx * ny;
     const int ncb = (int)((nxy + 255) / 256);
     const long want = (2048 + (long)ncb * nfield - 1) / ((long)ncb * nfield);
-    const int nch = (int)std::max(1L, std::min(want, (long)nz / 16));
+    const int nch = inplace ? 1 : (int)std::max(1L, std::min(want, (long)nz / 16));
     const int kc = (nz + nch - 1) / nch;
     const long nb = (long)ncb * ((nz + kc - 1) / kc) * nfield;
     if (nb >= (1L << 31)) return hipErrorInvalidValue;
@@ -192,9 +321,9 @@ hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, i
   return hipGetLastError();
 }
 template hipError_t launch_filter<double>(int, int, const double*, double*, int, int, int, long,
-                                          int, int, double, int, hipStream_t, const int*);
+                                          int, int, double, int, hipStream_t, const int*, bool);
 template hipError_t launch_filter<float>(int, int, const float*, float*, int, int, int, long, int,
-                                         int, float, int, hipStream_t, const int*);
+                                         int, float, int, hipStream_t, const int*, bool);
 
 // ---------------------------------------------------------------------------
 // Standalone derivatives (FP64, the reference's summation order)

@@ -161,8 +161,9 @@ class Server:
         self.setup_workers = max(1, setup_workers)
         self.poll_chunk = max(1, poll_chunk)
         self.chunk_dir = chunk_dir
-        # autotune: tile and z chunk tuned per variant group of each new
-        # membership (batch.tune_tiles; groups seen before reuse their choice)
+        # autotune: tile and z chunk tuned once per kernel variant (the first
+        # batch that holds it) and reused by every later membership it fits
+        # (batch.tune_tiles with by_variant)
         self.autotune = autotune
         self.clock = clock
 
@@ -235,7 +236,7 @@ class Server:
                         batch.close()
                     sol = [active[r]["solver"] for r in key]
                     tb = self.clock()
-                    batch = Batch(sol, use_graph=self.use_graph, autotune=self.autotune)
+                    batch = Batch(sol, use_graph=self.use_graph, autotune=self.autotune, tune_by_variant=True)
                     rebuild += self.clock() - tb
                     batch.time = [active[r]["done"] for r in key]
                     members = key
