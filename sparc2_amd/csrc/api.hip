@@ -35,8 +35,12 @@ hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, i
                          int nfield, int periodic, T s, int bare, hipStream_t st, const int* halt,
                          bool inplace = false);
 template <typename T>
-hipError_t launch_filter_xy(int g, const T* in, T* out, int nx, int ny, int nz, long fs, int nfield,
-                            int periodic, T s, hipStream_t st, const int* halt);
+hipError_t launch_filter_xy(int g, const void* in, void* out, int nx, int ny, int nz, long fs,
+                            int nfield, int periodic, T s, hipStream_t st, const int* halt, bool cbf);
+template <typename T>
+hipError_t launch_filter_zc(int g, const void* in, void* out, int nx, int ny, int nz, long fs,
+                            int nfield, T s, int bare, hipStream_t st, const int* halt, bool inplace,
+                            bool cbf);
 hipError_t launch_deriv(int axis, const double* in, double* out, int nx, int ny, int nz, double sxy,
                         const double* stretch, int bc, hipStream_t st);
 hipError_t launch_diag_reduce(const double* partial, int nblk, double* out, const GuardArgs& g,
@@ -191,6 +195,14 @@ static int download(sparc_ctx* c, double* dst, const void* src, long n) {
     HIPCHK(hipStreamSynchronize(c->st));
     for (long q = 0; q < n; ++q) dst[q] = (double)tmp[q];
   }
+  return SPARC_OK;
+}
+
+// n bfloat16 compensation terms from the device
+static int download_comp(sparc_ctx* c, uint16_t* dst, const void* src, long n) {
+  if (n <= 0) return SPARC_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(uint16_t) * n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
   return SPARC_OK;
 }
 
@@ -443,7 +455,7 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
     if ((rc = dalloc(&c->pool[p], sbytes))) return bail(rc);
   if (c->f32)
     for (int p = 0; p < 2; ++p)
-      if ((rc = dalloc(&c->comp[p], sbytes))) return bail(rc);
+      if ((rc = dalloc(&c->comp[p], sbytes / 2))) return bail(rc);  // bfloat16 terms
   if (c->zp)
     if ((rc = dalloc(&c->dzbuf, c->esz * 6 * (size_t)c->fs))) return bail(rc);
   const long bgn = c->bg3d ? c->fs : q.nz;
@@ -491,8 +503,8 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
     if (launch_zero(c->pool[p], (long)c->nv * c->fs, c->f32, c->st) != hipSuccess)
       return bail(fail(SPARC_E_HIP, "zero fill failed"));
   if (c->f32)
-    for (int p = 0; p < 2; ++p)
-      if (launch_zero(c->comp[p], (long)c->nv * c->fs, true, c->st) != hipSuccess)
+    for (int p = 0; p < 2; ++p)  // bfloat16 terms: half as many 4-byte words
+      if (launch_zero(c->comp[p], (long)c->nv * c->fs / 2, true, c->st) != hipSuccess)
         return bail(fail(SPARC_E_HIP, "zero fill failed"));
   if ((rc = upload(c, c->stretch, q.stretch, q.nz))) return bail(rc);
   if (c->f32) {
@@ -605,7 +617,7 @@ int sparc_hip_put_state(sparc_ctx* c, const double* a, const double* const psi[6
   HIPCHK(hipSetDevice(c->dev));
   if (c->f32)
     for (int p = 0; p < 2; ++p)
-      if (launch_zero(c->comp[p], (long)c->nv * c->fs, true, c->st) != hipSuccess)
+      if (launch_zero(c->comp[p], (long)c->nv * c->fs / 2, true, c->st) != hipSuccess)
         return fail(SPARC_E_HIP, "zero fill failed");
   if (a) {
     if (!c->f32) {
@@ -617,17 +629,18 @@ int sparc_hip_put_state(sparc_ctx* c, const double* a, const double* const psi[6
       // FP32: the value is held as hi - comp (the Kahan pair get_state
       // reads back as a - comp), so an FP64 state -- one written from an
       // FP32 run included -- comes back with its compensation:
-      // hi = fl32(x), comp = fl32(hi - x)
-      std::vector<float> hi(c->N), lo(c->N);
+      // hi = fl32(x), comp = bf16(fl32(hi - x))
+      std::vector<float> hi(c->N);
+      std::vector<uint16_t> lo(c->N);
       for (int f = 0; f < c->nv; ++f) {
         const double* x = a + (long)f * c->N;
         for (long q = 0; q < c->N; ++q) {
           hi[q] = (float)x[q];
-          lo[q] = (float)((double)hi[q] - x[q]);
+          lo[q] = f2bf((float)((double)hi[q] - x[q]));
         }
         HIPCHK(hipMemcpyAsync((char*)c->pool[c->ra] + 4 * (size_t)f * c->fs, hi.data(), 4 * c->N,
                               hipMemcpyHostToDevice, c->st));
-        HIPCHK(hipMemcpyAsync((char*)c->comp[c->rc] + 4 * (size_t)f * c->fs, lo.data(), 4 * c->N,
+        HIPCHK(hipMemcpyAsync((char*)c->comp[c->rc] + 2 * (size_t)f * c->fs, lo.data(), 2 * c->N,
                               hipMemcpyHostToDevice, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
       }
@@ -663,10 +676,10 @@ int sparc_hip_get_state(sparc_ctx* c, double* a, double* const psi[6]) {
       int rc = download(c, dst, (char*)c->pool[c->ra] + c->esz * (size_t)f * c->fs, c->N);
       if (rc) return rc;
       if (c->f32) {  // the compensated value is a - c
-        std::vector<double> cc(c->N);
-        rc = download(c, cc.data(), (char*)c->comp[c->rc] + 4 * (size_t)f * c->fs, c->N);
+        std::vector<uint16_t> cc(c->N);
+        rc = download_comp(c, cc.data(), (char*)c->comp[c->rc] + 2 * (size_t)f * c->fs, c->N);
         if (rc) return rc;
-        for (long q = 0; q < c->N; ++q) dst[q] -= cc[q];
+        for (long q = 0; q < c->N; ++q) dst[q] -= (double)bf2f(cc[q]);
       }
     }
   }
@@ -760,7 +773,7 @@ static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
   A.partial = c->partial;
   A.halt = c->guard;
   for (int f = 0; f < 8; ++f) {
-    A.comp[f] = c->f32 ? (T*)c->comp[c->rc] + (size_t)f * c->fs : nullptr;
+    A.comp[f] = c->f32 ? (uint16_t*)c->comp[c->rc] + (size_t)f * c->fs : nullptr;
     A.a[f] = (T*)c->pool[c->ra] + (size_t)f * c->fs;
   }
   for (int f = 0; f < 6; ++f) A.dz[f] = c->dzbuf ? (T*)c->dzbuf + (size_t)f * c->fs : nullptr;
@@ -833,24 +846,25 @@ int sparc_hip_step(sparc_ctx* c, double amp) {
 }
 
 // One filter pass out of place, then the buffer roles swap.  axis 2: z;
-// axis -1: the fused y-then-x pass.
+// axis -1: the fused y-then-x pass.  The FP32 compensation terms (bfloat16)
+// go through the same operator in FP32.
 template <typename T>
 static int filter_pass(sparc_ctx* c, int axis, int g, T s) {
-  auto launch = [&](const void* in, void* out) -> hipError_t {
+  auto launch = [&](const void* in, void* out, bool cbf) -> hipError_t {
     if (axis < 0)
-      return launch_filter_xy<T>(g, (const T*)in, (T*)out, c->nx, c->ny, c->nz, c->fs, c->nv,
-                                 c->cfg.periodic, s, c->st, c->guard);
-    return launch_filter<T>(axis, g, (const T*)in, (T*)out, c->nx, c->ny, c->nz, c->fs, c->nv,
-                            c->cfg.periodic, s, 0, c->st, c->guard);
+      return launch_filter_xy<T>(g, in, out, c->nx, c->ny, c->nz, c->fs, c->nv, c->cfg.periodic, s,
+                                 c->st, c->guard, cbf);
+    return launch_filter_zc<T>(g, in, out, c->nx, c->ny, c->nz, c->fs, c->nv, s, 0, c->st, c->guard,
+                               false, cbf);
   };
   TimeRec r;
   timed_begin(c, 3, &r);
-  hipError_t e = launch(c->pool[c->ra], c->pool[c->rk0]);
+  hipError_t e = launch(c->pool[c->ra], c->pool[c->rk0], false);
   timed_end(c, &r);
   if (e != hipSuccess) return fail(SPARC_E_HIP, "filter launch failed");
   std::swap(c->ra, c->rk0);
   if (c->f32) {  // the compensation term is filtered with the same linear operator
-    e = launch(c->comp[c->rc], c->comp[1 - c->rc]);
+    e = launch(c->comp[c->rc], c->comp[1 - c->rc], true);
     if (e != hipSuccess) return fail(SPARC_E_HIP, "filter launch failed");
     c->rc = 1 - c->rc;
   }
@@ -1259,20 +1273,21 @@ template <typename T>
 static int batch_filter(sparc_batch* b, sparc_ctx* c, int time) {
   const long n = (long)c->nv * c->fs;
   const bool xy = time % c->cfg.xyfreq == 0, z = time % c->cfg.zfreq == 0;
-  auto run = [&](void* a, void* t) -> hipError_t {
+  // cbf: the FP32 compensation terms (bfloat16, copied as n/2 4-byte words)
+  auto run = [&](void* a, void* t, bool cbf) -> hipError_t {
     hipError_t e = hipSuccess;
     if (xy)
-      e = launch_filter_xy<T>(c->cfg.g_xy_filt, (const T*)a, (T*)t, c->nx, c->ny, c->nz, c->fs, c->nv,
-                              c->cfg.periodic, (T)c->cfg.s_xy, b->st, c->guard);
+      e = launch_filter_xy<T>(c->cfg.g_xy_filt, a, t, c->nx, c->ny, c->nz, c->fs, c->nv,
+                              c->cfg.periodic, (T)c->cfg.s_xy, b->st, c->guard, cbf);
     if (e == hipSuccess && z)
-      e = launch_filter<T>(2, c->cfg.g_z_filt, (const T*)(xy ? t : a), (T*)a, c->nx, c->ny, c->nz,
-                           c->fs, c->nv, c->cfg.periodic, (T)c->cfg.s_z, 0, b->st, c->guard, !xy);
+      e = launch_filter_zc<T>(c->cfg.g_z_filt, xy ? t : a, a, c->nx, c->ny, c->nz, c->fs, c->nv,
+                              (T)c->cfg.s_z, 0, b->st, c->guard, !xy, cbf);
     else if (e == hipSuccess && xy)
-      e = launch_copy(t, a, n, c->f32, b->st, c->guard);
+      e = cbf ? launch_copy(t, a, n / 2, true, b->st, c->guard) : launch_copy(t, a, n, c->f32, b->st, c->guard);
     return e;
   };
-  hipError_t e = run(c->pool[c->ra], c->pool[c->rk0]);
-  if (e == hipSuccess && c->f32) e = run(c->comp[c->rc], c->comp[1 - c->rc]);
+  hipError_t e = run(c->pool[c->ra], c->pool[c->rk0], false);
+  if (e == hipSuccess && c->f32) e = run(c->comp[c->rc], c->comp[1 - c->rc], true);
   if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch filter: ") + hipGetErrorString(e));
   return SPARC_OK;
 }
@@ -1600,10 +1615,10 @@ int sparc_hip_get_slab(sparc_ctx* c, int field, int k0, int k1, double* out) {
   int rc = download(c, out, (const char*)c->pool[c->ra] + c->esz * off, n);
   if (rc) return rc;
   if (c->f32) {
-    std::vector<double> cc(n);
-    rc = download(c, cc.data(), (const char*)c->comp[c->rc] + 4 * off, n);
+    std::vector<uint16_t> cc(n);
+    rc = download_comp(c, cc.data(), (const char*)c->comp[c->rc] + 2 * off, n);
     if (rc) return rc;
-    for (long q = 0; q < n; ++q) out[q] -= cc[q];
+    for (long q = 0; q < n; ++q) out[q] -= (double)bf2f(cc[q]);
   }
   return SPARC_OK;
 }

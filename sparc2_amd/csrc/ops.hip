@@ -104,8 +104,19 @@ __global__ __launch_bounds__(256) void k_filter(const T* __restrict__ in, T* __r
 constexpr int FXY_KY = 64;
 constexpr int FXY_PD = 4;
 
-template <typename T, int G>
-__global__ __launch_bounds__(256) void k_filter_xy(const T* __restrict__ in, T* __restrict__ out,
+// S: the stored type -- T, or uint16_t for the FP32 form's bfloat16
+// compensation (filtered in FP32 with the same operator, stored rounded).
+template <typename S> __device__ __forceinline__ auto from_s(S v) {
+  if constexpr (sizeof(S) == 2) return bf2f(v);
+  else return v;
+}
+template <typename T, typename S> __device__ __forceinline__ S to_s(T v) {
+  if constexpr (sizeof(S) == 2) return f2bf(v);
+  else return v;
+}
+
+template <typename T, int G, typename S = T>
+__global__ __launch_bounds__(256) void k_filter_xy(const S* __restrict__ in, S* __restrict__ out,
                                                    int nx, int ny, int nz, long fs, int nseg, int nych,
                                                    long nwave, int periodic, T s,
                                                    const int* __restrict__ halt) {
@@ -129,16 +140,16 @@ __global__ __launch_bounds__(256) void k_filter_xy(const T* __restrict__ in, T* 
   const bool writer = lane >= H && lane < 64 - H && xl < nx;
   const int j0 = ych * FXY_KY, j1 = min(j0 + FXY_KY, ny);
   const long plane = (long)f * fs + (long)nx * ny * k;
-  const T* col = in + plane + (xin ? xg : 0);
-  T* ocol = out + plane + xl;
+  const S* col = in + plane + (xin ? xg : 0);
+  S* ocol = out + plane + xl;
   if (halted(halt)) {  // a run stopped by its failure guard keeps its state
     if (writer)
       for (int j = j0; j < j1; ++j) ocol[(long)j * nx] = col[(long)j * nx];
     return;
   }
   auto ld = [&](int r) -> T {
-    if (periodic) return col[(long)wrap(r, ny) * nx];
-    return (xin && r >= 0 && r < ny) ? col[(long)r * nx] : T(0);
+    if (periodic) return (T)from_s(col[(long)wrap(r, ny) * nx]);
+    return (xin && r >= 0 && r < ny) ? (T)from_s(col[(long)r * nx]) : T(0);
   };
   T q[Q], pf[FXY_PD];
 #pragma unroll
@@ -187,7 +198,7 @@ __global__ __launch_bounds__(256) void k_filter_xy(const T* __restrict__ in, T* 
     }
     T xf = acc;
     if (xcopy) xf = G == 5 ? sx : T(0);
-    if (writer) ocol[(long)j * nx] = G == 5 ? xf : sx - s * xf;
+    if (writer) ocol[(long)j * nx] = to_s<T, S>(G == 5 ? xf : sx - s * xf);
 #pragma unroll
     for (int m = 0; m < Q - 1; ++m) q[m] = q[m + 1];
   }
@@ -200,8 +211,8 @@ __global__ __launch_bounds__(256) void k_filter_xy(const T* __restrict__ in, T* 
 // Same per-point arithmetic, in the same order, as k_filter along z.
 constexpr int ZPD = 4;
 
-template <typename T, int G>
-__global__ __launch_bounds__(256) void k_filter_z(const T* in, T* out,
+template <typename T, int G, typename S = T>
+__global__ __launch_bounds__(256) void k_filter_z(const S* in, S* out,
                                                   int nx, int ny, int nz, long fs, int ncb, int kc,
                                                   T s, int bare, const int* __restrict__ halt) {
   const bool stop = halted(halt);  // guard stop: copy, as in k_filter
@@ -212,14 +223,14 @@ __global__ __launch_bounds__(256) void k_filter_z(const T* in, T* out,
   const long nxy = (long)nx * ny;
   const long col = (long)cb * 256 + threadIdx.x;
   if (col >= nxy) return;
-  const T* line = in + (long)f * fs + col;
-  T* o = out + (long)f * fs + col;
+  const S* line = in + (long)f * fs + col;
+  S* o = out + (long)f * fs + col;
   const int k0 = chunk * kc, k1 = min(k0 + kc, nz);
   if (stop) {
     for (int pos = k0; pos < k1; ++pos) o[(long)pos * nxy] = line[(long)pos * nxy];
     return;
   }
-  auto ld = [&](int r) -> T { return (r >= 0 && r < nz) ? line[(long)r * nxy] : T(0); };
+  auto ld = [&](int r) -> T { return (r >= 0 && r < nz) ? (T)from_s(line[(long)r * nxy]) : T(0); };
   constexpr int Q = 2 * G + 3;
   T q[Q], pf[ZPD];
 #pragma unroll
@@ -247,58 +258,89 @@ __global__ __launch_bounds__(256) void k_filter_z(const T* in, T* out,
       for (int m = -G; m <= G; ++m) acc += sm(m) * filt_w<T>(G, m);
       filt = acc;
     }
-    o[(long)pos * nxy] = (G == 5 || bare) ? filt : sm(0) - s * filt;
+    o[(long)pos * nxy] = to_s<T, S>((G == 5 || bare) ? filt : sm(0) - s * filt);
 #pragma unroll
     for (int m = 0; m < Q - 1; ++m) q[m] = q[m + 1];
   }
 }
 
+// z pass (FILTER_VARS, derivatives.f90:690-696) on stored type T or, cbf,
+// on bfloat16 compensation terms.  inplace: in may equal out -- one chunk per
+// column, whose register queue holds every raw value it still needs before
+// that plane is overwritten.
 template <typename T>
-hipError_t launch_filter_xy(int g, const T* in, T* out, int nx, int ny, int nz, long fs, int nfield,
-                            int periodic, T s, hipStream_t st, const int* halt) {
+hipError_t launch_filter_zc(int g, const void* in, void* out, int nx, int ny, int nz, long fs,
+                            int nfield, T s, int bare, hipStream_t st, const int* halt, bool inplace,
+                            bool cbf) {
+  if (cbf && sizeof(T) != 4) return hipErrorInvalidValue;
+  // column blocks x z chunks x fields: >= ~2048 blocks, chunks >= 16 deep
+  const long nxy = (long)nx * ny;
+  const int ncb = (int)((nxy + 255) / 256);
+  const long want = (2048 + (long)ncb * nfield - 1) / ((long)ncb * nfield);
+  const int nch = inplace ? 1 : (int)std::max(1L, std::min(want, (long)nz / 16));
+  const int kc = (nz + nch - 1) / nch;
+  const long nb = (long)ncb * ((nz + kc - 1) / kc) * nfield;
+  if (nb >= (1L << 31)) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)nb);
+#define SPARC_FZ(GG, S) \
+  hipLaunchKernelGGL((k_filter_z<T, GG, S>), grid, dim3(256), 0, st, (const S*)in, (S*)out, nx, ny, nz, fs, ncb, kc, s, bare, halt)
+  if (cbf) {
+    if (g == 5) SPARC_FZ(5, uint16_t);
+    else SPARC_FZ(3, uint16_t);
+  } else {
+    if (g == 5) SPARC_FZ(5, T);
+    else SPARC_FZ(3, T);
+  }
+#undef SPARC_FZ
+  return hipGetLastError();
+}
+template hipError_t launch_filter_zc<double>(int, const void*, void*, int, int, int, long, int, double,
+                                             int, hipStream_t, const int*, bool, bool);
+template hipError_t launch_filter_zc<float>(int, const void*, void*, int, int, int, long, int, float,
+                                            int, hipStream_t, const int*, bool, bool);
+
+// cbf (FP32 only): the arrays hold bfloat16 compensation terms.
+template <typename T>
+hipError_t launch_filter_xy(int g, const void* in, void* out, int nx, int ny, int nz, long fs,
+                            int nfield, int periodic, T s, hipStream_t st, const int* halt, bool cbf) {
   const int ow = 64 - 2 * (g + 1);
   const int nseg = (nx + ow - 1) / ow, nych = (ny + FXY_KY - 1) / FXY_KY;
   const long nwave = (long)nseg * nych * nz * nfield;
   const long nb = (nwave + 3) / 4;
   if (nb >= (1L << 31)) return hipErrorInvalidValue;
-  if (g == 5)
-    hipLaunchKernelGGL((k_filter_xy<T, 5>), dim3((unsigned)nb), dim3(256), 0, st, in, out, nx, ny, nz,
-                       fs, nseg, nych, nwave, periodic, s, halt);
-  else
-    hipLaunchKernelGGL((k_filter_xy<T, 3>), dim3((unsigned)nb), dim3(256), 0, st, in, out, nx, ny, nz,
-                       fs, nseg, nych, nwave, periodic, s, halt);
+  if (cbf && sizeof(T) != 4) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)nb);
+#define SPARC_FXY(GG, S)                                                                              \
+  hipLaunchKernelGGL((k_filter_xy<T, GG, S>), grid, dim3(256), 0, st, (const S*)in, (S*)out, nx, ny, \
+                     nz, fs, nseg, nych, nwave, periodic, s, halt)
+  if (cbf) {
+    if (g == 5) SPARC_FXY(5, uint16_t);
+    else SPARC_FXY(3, uint16_t);
+  } else {
+    if (g == 5) SPARC_FXY(5, T);
+    else SPARC_FXY(3, T);
+  }
+#undef SPARC_FXY
   return hipGetLastError();
 }
-template hipError_t launch_filter_xy<double>(int, const double*, double*, int, int, int, long, int,
-                                             int, double, hipStream_t, const int*);
-template hipError_t launch_filter_xy<float>(int, const float*, float*, int, int, int, long, int, int,
-                                            float, hipStream_t, const int*);
+template hipError_t launch_filter_xy<double>(int, const void*, void*, int, int, int, long, int, int,
+                                             double, hipStream_t, const int*, bool);
+template hipError_t launch_filter_xy<float>(int, const void*, void*, int, int, int, long, int, int,
+                                            float, hipStream_t, const int*, bool);
 
-// bare: the filtx/filty/filtz operators alone.  inplace (z only): in may
-// equal out -- one chunk per column, whose register queue holds every raw
-// value it still needs before that plane is overwritten.
+template <typename T>
+hipError_t launch_filter_zc(int g, const void* in, void* out, int nx, int ny, int nz, long fs,
+                            int nfield, T s, int bare, hipStream_t st, const int* halt, bool inplace,
+                            bool cbf);
+
+// One pass along x, y or z.  bare: the filtx/filty/filtz operators alone
+// (no edge smoothing); inplace: z only, see launch_filter_zc.
 template <typename T>
 hipError_t launch_filter(int axis, int g, const T* in, T* out, int nx, int ny, int nz, long fs,
                          int nfield, int periodic, T s, int bare, hipStream_t st, const int* halt,
                          bool inplace) {
   if (inplace && axis != 2) return hipErrorInvalidValue;
-  if (axis == 2) {
-    // column blocks x z chunks x fields: >= ~2048 blocks, chunks >= 16 deep
-    const long nxy = (long)nx * ny;
-    const int ncb = (int)((nxy + 255) / 256);
-    const long want = (2048 + (long)ncb * nfield - 1) / ((long)ncb * nfield);
-    const int nch = inplace ? 1 : (int)std::max(1L, std::min(want, (long)nz / 16));
-    const int kc = (nz + nch - 1) / nch;
-    const long nb = (long)ncb * ((nz + kc - 1) / kc) * nfield;
-    if (nb >= (1L << 31)) return hipErrorInvalidValue;
-    if (g == 5)
-      hipLaunchKernelGGL((k_filter_z<T, 5>), dim3((unsigned)nb), dim3(256), 0, st, in, out, nx, ny,
-                         nz, fs, ncb, kc, s, bare, halt);
-    else
-      hipLaunchKernelGGL((k_filter_z<T, 3>), dim3((unsigned)nb), dim3(256), 0, st, in, out, nx, ny,
-                         nz, fs, ncb, kc, s, bare, halt);
-    return hipGetLastError();
-  }
+  if (axis == 2) return launch_filter_zc<T>(g, in, out, nx, ny, nz, fs, nfield, s, bare, st, halt, inplace, false);
   // one block per x-row segment: 256 threads, or the row rounded up to
   // whole waves when it is shorter
   const int nt = nx >= 256 ? 256 : (nx + 63) / 64 * 64;

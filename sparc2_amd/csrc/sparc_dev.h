@@ -25,26 +25,46 @@ enum FaceId { XBOT = 0, XTOP = 1, YBOT = 2, YTOP = 3, ZBOT = 4, ZTOP = 5 };
 #define SPARC_DPC {0.872756993962, -0.286511173973, 0.090320001280, \
                    -0.020779405824, 0.002484594688}
 
-__constant__ static const double DPB_D[5][11] = {
-    {-2.391602219538, 5.832490322294, -7.650218001182 + 0.000000000001,
-     7.907810563576, -5.922599052629, 3.071037015445, -1.014956769726,
-     0.170022256519, 0.002819958377, -0.004791009708, -0.000013063429},
-    {-0.180022054228, -1.237550583044, 2.484731692990, -1.810320814061,
-     1.112990048440, -0.481086916514, 0.126598690230, -0.015510730165,
-     0.000024609059 - 0.000003, 0.000156447571 - 0.000000000001,
-     -0.000007390277},
-    {0.057982271137, -0.536135360383, -0.264089548967 + 0.000000000002,
-     0.917445877606 - 0.000000000002, -0.169688364841, -0.029716326170,
-     0.029681617641, -0.005222483773, -0.000118806260, -0.000118806260,
-     -0.000020069730},
-    {-0.013277273810, 0.115976072920, -0.617479187931,
-     -0.274113948206 + 0.000000000002, 1.086208764655 - 0.000000000002,
-     -0.402951626982, 0.131066986242, -0.028154858354, 0.002596328316,
-     0.000128743150, 0.0},
-    {0.016756572303, -0.117478455239, 0.411034935097, -1.130286765151,
-     0.341435872100 - 0.000000000001, 0.556396830543, -0.082525734207,
-     0.003565834658, 0.001173034777, -0.000071772671 + 0.000000000064,
-     -0.000000352273}};
+#define SPARC_DPB { \
+    {-2.391602219538, 5.832490322294, -7.650218001182 + 0.000000000001, \
+    7.907810563576, -5.922599052629, 3.071037015445, -1.014956769726, \
+    0.170022256519, 0.002819958377, -0.004791009708, -0.000013063429}, \
+    {-0.180022054228, -1.237550583044, 2.484731692990, -1.810320814061, \
+    1.112990048440, -0.481086916514, 0.126598690230, -0.015510730165, \
+    0.000024609059 - 0.000003, 0.000156447571 - 0.000000000001, \
+    -0.000007390277}, \
+    {0.057982271137, -0.536135360383, -0.264089548967 + 0.000000000002, \
+    0.917445877606 - 0.000000000002, -0.169688364841, -0.029716326170, \
+    0.029681617641, -0.005222483773, -0.000118806260, -0.000118806260, \
+    -0.000020069730}, \
+    {-0.013277273810, 0.115976072920, -0.617479187931, \
+    -0.274113948206 + 0.000000000002, 1.086208764655 - 0.000000000002, \
+    -0.402951626982, 0.131066986242, -0.028154858354, 0.002596328316, \
+    0.000128743150, 0.0}, \
+    {0.016756572303, -0.117478455239, 0.411034935097, -1.130286765151, \
+    0.341435872100 - 0.000000000001, 0.556396830543, -0.082525734207, \
+    0.003565834658, 0.001173034777, -0.000071772671 + 0.000000000064, \
+    -0.000000352273}}
+
+__constant__ static const double DPB_D[5][11] = SPARC_DPB;
+
+// The rows as pairs of floats, hi = fl32(b) and lo = fl32(b - hi), for the
+// FP32 form (see RowSum): formed at compile time, so a term costs two table
+// reads and no conversions.
+struct DpbF {
+  float hi[5][11], lo[5][11];
+};
+constexpr DpbF make_dpbf() {
+  constexpr double d[5][11] = SPARC_DPB;
+  DpbF t{};
+  for (int r = 0; r < 5; ++r)
+    for (int m = 0; m < 11; ++m) {
+      t.hi[r][m] = (float)d[r][m];
+      t.lo[r][m] = (float)(d[r][m] - (double)t.hi[r][m]);
+    }
+  return t;
+}
+__constant__ static const DpbF DPB_F = make_dpbf();
 
 template <typename T> struct Coef;
 template <> struct Coef<double> {
@@ -140,7 +160,7 @@ template <typename T> struct StageArgs {
   const T* u[8];
   T* a[8];
   T* un[8];
-  T* comp[8];  // Kahan compensation (fp32 only), updated in place on the last stage
+  uint16_t* comp[8];  // Kahan compensation (fp32 only, bfloat16), updated in place on the last stage
   T* dz[6];    // MHD z-pass output: z derivatives in CPML family order
   const T* damp[3];  // FFT damping terms of vx, vy, vz (null: no damping)
   // PULSE = 0 driver: this step's forcing(x, y) (nx*ny) and source_dep(z);
@@ -270,29 +290,38 @@ __device__ __forceinline__ F2& operator+=(F2& a, F2 b) { return a = a + b; }
 template <typename T, int VW> struct Lane { using V = T; };
 template <> struct Lane<float, 2> { using V = F2; };
 
-// One-sided boundary row sum_m b_m v_m, with the row's FP64 coefficients b_m
-// (derivatives.f90:20-30: the five rows next to a non-periodic x or y face
-// and below the top in z).  FP64: multiply-adds in row order.  FP32: these
-// coefficients reach 7.9 in magnitude and sum to zero, so a plain sum's
-// rounding error is ~30 ulp of the values, and over thousands of steps it,
-// not the centred rows, sets the FP32 error next to those faces (where the
-// flux-tube runs carry their largest velocities).  The sum is therefore
-// compensated -- Ogita, Rump & Oishi (2005), Dot2: each product split
-// exactly with an FMA, the coefficient's FP64 rest taken as one more
-// product, each addition split exactly with TwoSum -- which is as accurate
-// as the plain sum in twice the working precision, rounded once.
+// The FP32 form's Kahan compensation is stored as bfloat16: the upper half
+// of the float's bits, rounded to nearest even.  The compensation is the
+// rounding error of the FP32 state (at most half its ulp), so 8 significant
+// bits of it keep the pair (a, c) to ~2^-33 of |a| -- far below what FP32
+// arithmetic in the right-hand side already leaves -- with FP32's exponent
+// range, at half the bytes on the last RK stage and in the filters.
+__host__ __device__ inline uint16_t f2bf(float x) {
+  uint32_t u = __builtin_bit_cast(uint32_t, x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
+// One-sided boundary row sum_m b(r, m) v_m (derivatives.f90:20-30: the five
+// rows next to a non-periodic x or y face and below the top in z).  FP64:
+// multiply-adds in row order.  FP32: the rows' coefficients reach 7.9 in
+// magnitude and sum to zero, so each rounded coefficient's error, the same
+// at every step, adds up coherently over thousands of steps: in the small
+// flux-tube runs, whose edges are a large share of the grid, it set the
+// FP32 error (population run 48x48x216 after 2000 steps: 1.4e-5 relative L2
+// with float coefficients, 3.7e-6 with the pairs).  Each coefficient is
+// therefore the pair hi + lo of its FP64 value, with two accumulators, as
+// for the centred rows.  (A fully compensated sum -- Dot2, every product and
+// addition split exactly -- came out no more accurate here and slower.)
 template <typename T> struct RowSum {
   T p = 0, s = 0;
-  __device__ __forceinline__ void add(T v, double b) {
+  __device__ __forceinline__ void add(T v, int r, int m) {
     if constexpr (sizeof(T) == 8) {
-      p += v * b;
+      p += v * DPB_D[r][m];
     } else {
-      const float bh = (float)b, bl = (float)(b - (double)bh);
-      const float h = bh * v;
-      const float e = __builtin_fmaf(bh, v, -h) + bl * v;
-      const float t = p + h, z = t - p;
-      s += ((p - (t - z)) + (h - z)) + e;
-      p = t;
+      p += v * DPB_F.hi[r][m];
+      s += v * DPB_F.lo[r][m];
     }
   }
   __device__ __forceinline__ T sum() const {
@@ -302,9 +331,9 @@ template <typename T> struct RowSum {
 };
 struct RowSumF2 {
   RowSum<float> a, b;
-  __device__ __forceinline__ void add(F2 v, double c) {
-    a.add(v.x, c);
-    b.add(v.y, c);
+  __device__ __forceinline__ void add(F2 v, int r, int m) {
+    a.add(v.x, r, m);
+    b.add(v.y, r, m);
   }
   __device__ __forceinline__ F2 sum() const { return F2(a.sum(), b.sum()); }
 };
@@ -344,6 +373,32 @@ __device__ __forceinline__ void stv(const Buf<T>& b, uint32_t e, V v) {
   } else {
     static_assert(sizeof(V) == 4, "a lane value of 4 or 8 bytes");
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), b.r, (int)(e * 4u), 0, CP);
+  }
+}
+
+// Compensation (bfloat16) of the lane's point(s): V = float (one point, a
+// 2-byte access) or F2 (two points, 4 bytes).  ldcomp takes the byte offset
+// of the lane's first point in a float field, halved here (Buf<float>::kOut
+// stays past the range); stcomp the element index.
+template <typename V, int CP = 0>
+__device__ __forceinline__ V ldcomp(const Buf<uint16_t>& b, uint32_t fbo) {
+  const int bo = (int)(fbo == Buf<float>::kOut ? fbo : fbo >> 1);
+  if constexpr (sizeof(V) == 8) {
+    const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(b.r, bo, 0, CP);
+    return V(bf2f((uint16_t)(u & 0xFFFFu)), bf2f((uint16_t)(u >> 16)));
+  } else {
+    static_assert(sizeof(V) == 4, "a compensation lane of one or two FP32 points");
+    return bf2f(__builtin_amdgcn_raw_buffer_load_b16(b.r, bo, 0, CP));
+  }
+}
+template <int CP = 0, typename V>
+__device__ __forceinline__ void stcomp(const Buf<uint16_t>& b, uint32_t e, V v) {
+  if constexpr (sizeof(V) == 8) {
+    const uint32_t u = (uint32_t)f2bf(v.x) | ((uint32_t)f2bf(v.y) << 16);
+    __builtin_amdgcn_raw_buffer_store_b32(u, b.r, (int)(e * 2u), 0, CP);
+  } else {
+    static_assert(sizeof(V) == 4, "a compensation lane of one or two FP32 points");
+    __builtin_amdgcn_raw_buffer_store_b16(f2bf(v), b.r, (int)(e * 2u), 0, CP);
   }
 }
 

@@ -264,12 +264,12 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         RowSum<T> acc;
         if (p < 5) {
 #pragma unroll
-          for (int m = 0; m < 11; ++m) acc.add(rp[min(m - i0 + XO, RW - 1)], DPB_D[p][m]);
+          for (int m = 0; m < 11; ++m) acc.add(rp[min(m - i0 + XO, RW - 1)], p, m);
           set_lane_pt(out, c, scl(sxs, sxl, acc.sum()));
         } else if (p >= nx - 5) {
           const int r = nx - 1 - p;
 #pragma unroll
-          for (int m = 0; m < 11; ++m) acc.add(rp[min(max(nx - 1 - m - i0 + XO, 0), RW - 1)], DPB_D[r][m]);
+          for (int m = 0; m < 11; ++m) acc.add(rp[min(max(nx - 1 - m - i0 + XO, 0), RW - 1)], r, m);
           set_lane_pt(out, c, -scl(sxs, sxl, acc.sum()));
         }
       }
@@ -281,14 +281,14 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     if (!periodic && j < 5) {
       typename RowSumOf<V>::type e;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) e.add(col[min(max(m - j0 + 5, 0), LY - 1) * LX], DPB_D[j][m]);
+      for (int m = 0; m < 11; ++m) e.add(col[min(max(m - j0 + 5, 0), LY - 1) * LX], j, m);
       return scl(sys, syl, e.sum());
     }
     if (!periodic && j >= ny - 5) {
       const int r = ny - 1 - j;
       typename RowSumOf<V>::type e;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) e.add(col[min(max(ny - 1 - m - j0 + 5, 0), LY - 1) * LX], DPB_D[r][m]);
+      for (int m = 0; m < 11; ++m) e.add(col[min(max(ny - 1 - m - j0 + 5, 0), LY - 1) * LX], r, m);
       return -scl(sys, syl, e.sum());
     }
     V acc = 0;
@@ -378,7 +378,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       if (last) {
         if constexpr (NPC > 0) {
 #pragma unroll
-          for (int f = 0; f < NV; ++f) dst[IC + f] = LDS1(Ap->comp[f], bo);
+          for (int f = 0; f < NV; ++f) dst[IC + f] = ldcomp<V, CPL>(Buf<uint16_t>(Ap->comp[f], fs), bo);
         }
         if constexpr (NPP > 0) dst[IP] = LDS1(Ap->p0, bo);
       }
@@ -414,7 +414,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         pm[MP_P0] = LDS1(Ap->p0, bo);
         if constexpr (sizeof(T) == 4) {
 #pragma unroll
-          for (int f = 0; f < 8; ++f) pm[MP_C + f] = LDS1(Ap->comp[f], bo);
+          for (int f = 0; f < 8; ++f) pm[MP_C + f] = ldcomp<V, CPL>(Buf<uint16_t>(Ap->comp[f], fs), bo);
         }
       }
     }
@@ -484,7 +484,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           for (int f = 0; f < NQ; ++f) {
             typename RowSumOf<V>::type acc;
 #pragma unroll
-            for (int m = 0; m < 11; ++m) acc.add(q[f][m], DPB_D[r][(r + 16 - m) % 11]);
+            for (int m = 0; m < 11; ++m) acc.add(q[f][m], r, (r + 16 - m) % 11);
             dz[f] = -scl(st, stl, acc.sum());
           }
         }
@@ -715,14 +715,14 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           const V a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : ldv<V>(af, g * W));
           if constexpr (sizeof(T) == 4) {
             // compensated update: a - c carries the bits a + dt K loses
-            const Buf<T> cf(Ap->comp[f], fs);
+            const Buf<uint16_t> cf(Ap->comp[f], fs);
             V c0;
             if constexpr (NPC > 0) c0 = pc[IC + f];
             else if constexpr (MPX) c0 = pm[MP_C + f];
-            else c0 = ldv<V>(cf, g * W);
+            else c0 = ldcomp<V>(cf, g * W);
             const V y = scl(dt, Ap->dt_lo, K[f]) - c0;
             const V t = a0 + y;
-            stv<CPW>(cf, g, (t - a0) - y);
+            stcomp<CPW>(cf, g, (t - a0) - y);
             an[f] = t;
           } else {
             an[f] = a0 + dt * K[f];
@@ -876,7 +876,7 @@ __device__ __forceinline__ void zpass_item(CArgs<T> Ap, const int logical) {
       for (int f = 0; f < 6; ++f) {
         RowSum<T> acc;
 #pragma unroll
-        for (int m = 0; m < 11; ++m) acc.add(q[f][m], DPB_D[r][(r + 16 - m) % 11]);
+        for (int m = 0; m < 11; ++m) acc.add(q[f][m], r, (r + 16 - m) % 11);
         d[f] = -scl(st, stl, acc.sum());
       }
     }

@@ -64,11 +64,12 @@ class Result:
 
 
 def device_footprint(cfg: RunConfig) -> int:
-    """Bytes a run keeps on the device: 3 state buffers (+2 compensation
-    buffers in FP32), backgrounds, CPML arrays."""
+    """Bytes a run keeps on the device: 3 state buffers (+2 bfloat16
+    compensation buffers in FP32, each half a state buffer), backgrounds,
+    CPML arrays."""
     w = 4 if cfg.precision == "fp32" else 8
     fs = -(-cfg.npoints // 256) * 256
-    n_state = 3 + (2 if cfg.precision == "fp32" else 0)
+    n_state = 3 + (1 if cfg.precision == "fp32" else 0)
     n_bg = 18 if cfg.mhd else 7
     psi = sum(int(np.prod(cfg.face_shape(f))) for f in range(6) if cfg.face_active(f))
     return w * (n_state * cfg.nv * fs + n_bg * fs + psi * (3 + 2 * cfg.npsi)) + (1 << 20)
