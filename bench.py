@@ -69,13 +69,15 @@ def alg_bytes_xy(cfg, w):
 
 def alg_bytes_step(cfg, w):
     """B/pt x N for a whole step: 5 stages + amortised filters (+ the Kahan
-    term in the FP32 form)."""
+    term in the FP32 form: read and written once per step, 2 bytes per value
+    -- the compensation is stored as bfloat16; a 4-byte term would add
+    2 nv x 2 B/pt more)."""
     N = cfg.npoints
     nv = cfg.nv
     per_pt = 15 * nv + 5 * n_background(cfg) + 2 * nv * (1.0 / cfg.zfreq + 1.0 / cfg.xyfreq)
     b = w * (N * per_pt + 15 * cfg.npsi * psi_points(cfg))
     if cfg.precision == "fp32":
-        b += 2 * nv * 4 * N
+        b += 2 * nv * 2 * N
     return b
 
 
@@ -268,16 +270,23 @@ def mode_serve(args):
 
 
 def mode_profile(args):
-    """What rocprofv3 runs: one device copy of 1 GiB (its FETCH_SIZE /
-    WRITE_SIZE reading calibrates the counters against a known byte count),
-    then --steps driver steps of the workload.  No timing of its own."""
+    """What rocprofv3 runs: one device copy of 1 GiB and two launches of the
+    library's stream kernel (4 arrays of 512 MiB read, one written) -- their
+    FETCH_SIZE / WRITE_SIZE readings calibrate the counters against known
+    byte counts of two access patterns -- then --steps driver steps of the
+    workload.  No timing of its own."""
+    import ctypes as C
+
     import torch
+    from sparc2_amd._lib import check, lib
     a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
     b.copy_(a)
     torch.cuda.synchronize()
     del a, b
     torch.cuda.empty_cache()
+    g = C.c_double()
+    check(lib().sparc_hip_stream_bench(0, 1 << 26, 4, 2, C.byref(g)), "sparc_hip_stream_bench")
     from sparc2_amd import workloads
     from sparc2_amd.solver import Solver
     cfg, bg = workloads.build(args.workload, precision=args.precision)
@@ -286,7 +295,8 @@ def mode_profile(args):
     s.advance(1, args.steps)
     s.sync()
     print(json.dumps({"mode": "profile", "workload": args.workload, "precision": args.precision,
-                      "steps": args.steps, "tile": list(s.tile()[0]), "copy_bytes": 1 << 30}),
+                      "steps": args.steps, "tile": list(s.tile()[0]), "copy_bytes": 1 << 30,
+                      "stream_read_bytes": 4 << 29, "stream_write_bytes": 1 << 29}),
           flush=True)
     s.close()
 

@@ -120,9 +120,10 @@ int sparc_hip_set_forcing(sparc_ctx* ctx, const double* map, const double* sourc
  * PML bases zpmlvars/xpmlvars/ypmlvars), NULL for inactive faces or to leave
  * them untouched (put) / skip them (get).  put_state clears the stage buffers.
  * In FP32 a state value is held as a pair (v, c) meaning v - c (the Kahan
- * compensation of the RK update): get_state returns v - c in FP64 and
- * put_state splits x into v = fl32(x), c = fl32(v - x), so a state written
- * from an FP32 run and read back continues as the uninterrupted run. */
+ * compensation of the RK update, stored as bfloat16): get_state returns
+ * v - c in FP64 and put_state splits x into v = fl32(x), c = bf16(fl32(v - x)),
+ * so a state written from an FP32 run and read back continues as the
+ * uninterrupted run (to ~2^-33 of the value). */
 int sparc_hip_put_state(sparc_ctx* ctx, const double* a, const double* const psi[6]);
 int sparc_hip_get_state(sparc_ctx* ctx, double* a, double* const psi[6]);
 
