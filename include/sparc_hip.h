@@ -252,8 +252,8 @@ int sparc_hip_batch_destroy(sparc_batch* b);
 int sparc_hip_batch_time_stage(sparc_batch* b, int reps, double* ms_out);
 
 /* Bandwidth probe (no counterpart in the reference): reps launches of a
- * kernel reading nread arrays of n doubles and writing their sum to one
- * more (16-byte non-temporal accesses), on device `device`; gbps = bytes
+ * kernel reading nread arrays (1..48) of n doubles and writing their sum to
+ * one more (16-byte non-temporal accesses), on device `device`; gbps = bytes
  * moved per second / 1e9.  It allocates its own buffers.  The stage
  * kernels' roofline fractions are read against it at their read-to-write
  * ratios as well as against the data-sheet HBM peak. */

@@ -1627,7 +1627,7 @@ int sparc_hip_get_slab(sparc_ctx* c, int field, int k0, int k1, double* out) {
 // Measured bandwidth ceiling (no counterpart in the reference)
 // ===========================================================================
 int sparc_hip_stream_bench(int device, long n, int nread, int reps, double* gbps) {
-  if (!gbps || n < 2 || n % 2 || nread < 1 || nread > 16 || reps < 1)
+  if (!gbps || n < 2 || n % 2 || nread < 1 || nread > 48 || reps < 1)
     return fail(SPARC_E_INVALID, "bad argument");
   HIPCHK(hipSetDevice(device));
   std::vector<double*> buf(nread + 1, nullptr);

@@ -602,7 +602,7 @@ namespace sparc {
 typedef double sd2 __attribute__((ext_vector_type(2)));
 
 struct StreamArgs {
-  const sd2* in[16];
+  const sd2* in[48];
   sd2* out;
   long n2;  // 16-byte elements per array
   int nread;
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(256) void k_stream(StreamArgs A) {
 
 hipError_t launch_stream(const double* const* in, int nread, double* out, long n, int nblk,
                          hipStream_t st) {
-  if (nread < 1 || nread > 16 || n % 2) return hipErrorInvalidValue;
+  if (nread < 1 || nread > 48 || n % 2) return hipErrorInvalidValue;
   StreamArgs A{};
   for (int r = 0; r < nread; ++r) A.in[r] = (const sd2*)in[r];
   A.out = (sd2*)out;
