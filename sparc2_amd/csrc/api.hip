@@ -124,6 +124,9 @@ struct sparc_ctx {
   // MHD xy kernel; SPARC_LANE_POINTS=1 keeps one), else 1.  Tile widths
   // count lanes, so a tile spans vw*tx points.
   int vw = 1;
+  // FP64 MHD xy kernel: stage with direct-to-LDS loads where the tile is
+  // 16x16 and nx even (SPARC_MHD_DMA=0 keeps the register staging)
+  bool dma = false;
   void* dzbuf = nullptr;
   void* bg[B_COUNT] = {};
   void *g = nullptr, *stretch = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
@@ -420,6 +423,8 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
     c->zp = c->mhd && !(fused && fused[0] == '1');
     const char* lp = getenv("SPARC_LANE_POINTS");
     c->vw = (c->f32 && q.nx % 2 == 0 && (c->zp || !c->mhd) && !(lp && lp[0] == '1')) ? 2 : 1;
+    const char* dm = getenv("SPARC_MHD_DMA");
+    c->dma = !c->f32 && c->zp && q.nx % 2 == 0 && !(dm && dm[0] == '0');
   }
   c->nv = c->mhd ? 8 : 5;
   c->npsi = c->mhd ? 6 : 2;
@@ -720,6 +725,7 @@ static void fill_common(sparc_ctx* c, StageArgs<T>& A) {
   A.nchunk = c->nchunk;
   A.zkc = c->zkc;
   A.znchunk = c->znchunk;
+  A.dma = c->dma && c->tx == 16 && c->ty == 16;
   A.periodic = c->cfg.periodic ? 1 : 0;
   A.pulse_dir = c->cfg.pulse_dir;
   A.rinv_rho0 = c->rinv_rho0 ? 1 : 0;

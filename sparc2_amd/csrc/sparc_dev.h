@@ -149,6 +149,9 @@ template <typename T> struct StageArgs {
   // FP64 MHD xy kernel divides (correctly rounded, the same bits) instead of
   // reading the array
   int rinv_rho0;
+  // FP64 MHD xy kernel on 16x16 tiles with even nx: stage the planes with
+  // direct-to-LDS buffer loads (k_stage_dma)
+  int dma;
   T beta_next, dt, sxs, sys, src_amp;
   // FP32: the remainders of the FP64 values (scl above); 0 in FP64
   T beta_lo, dt_lo, sxs_lo, sys_lo;
@@ -374,6 +377,14 @@ __device__ __forceinline__ void stv(const Buf<T>& b, uint32_t e, V v) {
     static_assert(sizeof(V) == 4, "a lane value of 4 or 8 bytes");
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), b.r, (int)(e * 4u), 0, CP);
   }
+}
+
+// 16 bytes per lane from a buffer straight into LDS, lane l writing bytes
+// lds + 16 l .. lds + 16 l + 15 (gfx950 buffer_load_dwordx4 with lds; no
+// registers).  lds is uniform over the wave and points into shared memory.
+// Counted by vmcnt like any load.
+__device__ __forceinline__ void lds_load16(__amdgpu_buffer_rsrc_t r, void* lds, uint32_t voff) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16, voff, 0, 0, 0);
 }
 
 // Compensation (bfloat16) of the lane's point(s): V = float (one point, a

@@ -42,22 +42,26 @@ constexpr int stage_waves() {
 // CPML-corrected on the z faces, come from the z pass (Ap->dz) instead of the
 // register queue.  VW: x points per lane (2: FP32 on even nx, F2 values); the
 // tile is TX lanes = VW*TX points wide.
-template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP, int VW>
+template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP, int VW, bool DMA = false>
 __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   static_assert(MHD || !ZP, "the z pass serves the MHD stage only");
+  static_assert(!DMA || (MHD && ZP && VW == 1 && sizeof(T) == 8 && TX == 16 && TY == 16),
+                "direct-to-LDS staging: the FP64 MHD xy kernel on 16x16 tiles");
   static_assert(VW == 1 || (sizeof(T) == 4 && (ZP || !MHD)), "two-point lanes: FP32 quiet or MHD xy");
   using V = typename Lane<T, VW>::V;
   constexpr int NT = TX * TY;
-  // x halo in lane columns: 5 points, rounded up to whole lanes
-  constexpr int HX = VW == 1 ? 5 : 3;
+  // x halo in lane columns: 5 points, rounded up to whole lanes (DMA: 6, so
+  // the 16-byte cell pairs of a row never straddle a periodic wrap)
+  constexpr int HX = DMA ? 6 : (VW == 1 ? 5 : 3);
   constexpr int LX = TX + 2 * HX, LY = TY + 10;
   constexpr int NR = TY * LX + 10 * TX;  // cross-shaped staging region
   constexpr int NS = (NR + NT - 1) / NT;
   constexpr int NQ = ZP ? 1 : (MHD ? 6 : 2);
   constexpr int NV = MHD ? 8 : 5;
   // LDS planes: 0 vx (x), 1 vy (y), 2 p (xy); MHD: 3 F3 (xy), 4 F2 (x),
-  // 5 F1 (y), 6 by (x), 7 bz (xy), 8 bx (y)
-  constexpr int NL = MHD ? 9 : 3;
+  // 5 F1 (y), 6 by (x), 7 bz (xy), 8 bx (y).  DMA: planes 3-5 arrive as
+  // box, boy, boz and 9 as vz, and the flux pass turns 3-5 into the fluxes.
+  constexpr int NL = MHD ? (DMA ? 10 : 9) : 3;
   // SPF (quiet): two LDS plane buffers, the next plane's staging loads held
   // in registers while the current plane computes (one barrier per plane).
   // MHD: one buffer, loads then a barrier per plane -- with nine staged
@@ -67,8 +71,10 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   constexpr bool SPF = !MHD;
   constexpr int NB = SPF ? 2 : 1;
   constexpr int NPRE = MHD ? 10 : 3;  // raw values staged per cell
-  // one spare cell per plane takes the writes of slots past the region
-  constexpr int LP = LY * LX + 1;
+  // one spare cell per plane takes the writes of slots past the region.
+  // DMA: the cross-shaped region packed (band rows of LX cells, then the
+  // ten halo rows of TX cells), padded to whole 128-cell wave loads.
+  constexpr int LP = DMA ? (NR + 127) / 128 * 128 : LY * LX + 1;
   __shared__ V LB[NB][NL][LP];
   __shared__ double rs[3][NT / 64 + 1];
 
@@ -124,6 +130,43 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     s_ok[s] = ok;
     s_off[s] = ok ? (uint32_t)(VW * gi + nx * gj) : 0u;
     s_lds[s] = r < NR ? row * LX + col : LY * LX;
+  }
+
+  // DMA plan: one wave load moves 64 x 16 bytes -- two adjacent x cells per
+  // lane -- into 128 consecutive packed LDS cells; chunk c of a plane is
+  // packed cells c*128 .. c*128+127.  Per lane and chunk: the element offset
+  // of its cell pair in the plane, or ~0 (the load reads 0: outside the
+  // domain of a non-periodic axis, or past the region).  With nx even, HX
+  // even and the tile's first x a multiple of 16, a pair starts at an even
+  // x, so it lies wholly inside or outside the domain and never straddles
+  // a periodic wrap.
+  constexpr int NCH = DMA ? LP / 128 : 1;
+  uint32_t d_off[NCH];
+  if constexpr (DMA) {
+    const int lane = tid & 63;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int r = c * 128 + 2 * lane;
+      int row, col;
+      if (r < TY * LX) {
+        row = 5 + r / LX;
+        col = r % LX;
+      } else {
+        const int r2 = r - TY * LX;
+        const int hr = r2 / TX;
+        row = hr < 5 ? hr : hr + TY;
+        col = HX + r2 % TX;
+      }
+      int gi = i0 - HX + col, gj = j0 - 5 + row;
+      bool ok = r < NR;
+      if (periodic) {
+        gi = wrap(gi, nx);
+        gj = wrap(gj, ny);
+      } else {
+        ok = ok && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+      }
+      d_off[c] = ok ? (uint32_t)(gi + nx * gj) : 0xFFFFFFFFu;
+    }
   }
 
   // Array views, rebuilt from the argument pointer where they are used.
@@ -190,6 +233,28 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       }
     }
   };
+  // DMA: the ten raw planes of staging row data, straight to LDS (no
+  // registers), the loads spread over the block's waves
+  auto dma_plane = [&](int kk) {
+    if constexpr (DMA) {
+      const uint32_t pk = (uint32_t)kk * nxy;
+      uint32_t bo[NCH];
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) bo[c] = d_off[c] == 0xFFFFFFFFu ? Buf<T>::kOut : (pk + d_off[c]) * W;
+      const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+      const T* const src[NL] = {Ap->u[VX], Ap->u[VY], Ap->u[P], Ap->box, Ap->boy,
+                                Ap->boz,   Ap->u[BY], Ap->u[BZ], Ap->u[BX], Ap->u[VZ]};
+#pragma unroll
+      for (int f = 0; f < NL; ++f) {
+        const Buf<T> b(src[f], fs);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          if ((f * NCH + c) % (NT / 64) != wave) continue;  // uniform over the wave
+          lds_load16(b.r, &LB[0][f][c * 128], bo[c]);
+        }
+      }
+    }
+  };
   auto stage_plane = [&](int k) {
     const uint32_t pk = (uint32_t)k * nxy;
 #pragma unroll
@@ -212,12 +277,22 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       }
     }
   };
+  // LDS index of (staging row, column): the full rectangle, or with DMA the
+  // packed cross (rows 5..TY+4 are the band, the rest halo rows of the tile's
+  // own TX columns -- the only cells the y derivatives read there)
+  auto lidx = [&](int row, int col) -> int {
+    if constexpr (!DMA) return row * LX + col;
+    else {
+      if (row >= 5 && row < 5 + TY) return (row - 5) * LX + col;
+      return TY * LX + (row < 5 ? row : row - TY) * TX + (col - HX);
+    }
+  };
   // 11-point derivative along x / y from LDS (derivatives.f90:38-44,66-75,92-101).
   // x: the centred row for every point of the lane, then the one-sided rows
   // for points within 5 of a non-periodic edge, reading the LDS row as
   // points (point x sits at index x - i0 + VW*HX).
   auto dx_lds = [&](int f) -> V {
-    const V* row = &L[f][(ty + 5) * LX];
+    const V* row = &L[f][lidx(ty + 5, 0)];
     V out;
     if constexpr (VW == 1) {
       T acc = 0, acl = 0;
@@ -277,18 +352,19 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     return out;
   };
   auto dy_lds = [&](int f) -> V {
-    const V* col = &L[f][tx + HX];
+    const V* pl = L[f];
+    const int cx = tx + HX;
     if (!periodic && j < 5) {
       typename RowSumOf<V>::type e;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) e.add(col[min(max(m - j0 + 5, 0), LY - 1) * LX], j, m);
+      for (int m = 0; m < 11; ++m) e.add(pl[lidx(min(max(m - j0 + 5, 0), LY - 1), cx)], j, m);
       return scl(sys, syl, e.sum());
     }
     if (!periodic && j >= ny - 5) {
       const int r = ny - 1 - j;
       typename RowSumOf<V>::type e;
 #pragma unroll
-      for (int m = 0; m < 11; ++m) e.add(col[min(max(ny - 1 - m - j0 + 5, 0), LY - 1) * LX], r, m);
+      for (int m = 0; m < 11; ++m) e.add(pl[lidx(min(max(ny - 1 - m - j0 + 5, 0), LY - 1), cx)], r, m);
       return -scl(sys, syl, e.sum());
     }
     V acc = 0;
@@ -296,7 +372,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     V acl = 0;
 #pragma unroll
     for (int m = 1; m <= 5; ++m) {
-      const V d = col[(r0 + m) * LX] - col[(r0 - m) * LX];
+      const V d = pl[lidx(r0 + m, cx)] - pl[lidx(r0 - m, cx)];
       acc += Coef<T>::c(m - 1) * d;
       if constexpr (sizeof(T) == 4) acl += Coef<T>::cl(m - 1) * d;
     }
@@ -445,6 +521,24 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         pre_load(k + 1);
         if constexpr (PPF) point_load(k + 1, pn);
       }
+    } else if constexpr (MPX && DMA) {
+      dma_plane(k);
+      __builtin_amdgcn_sched_barrier(0);  // the plane's LDS loads stay older than the point loads
+      mpoint_load(k);
+      // all but the 33 youngest loads (the point values: 33, 34 on the last
+      // stage) done: the plane's LDS loads have landed.  0x8F71: vmcnt 33,
+      // expcnt and lgkmcnt not waited on.
+      __builtin_amdgcn_s_waitcnt(0x8F71);
+      __syncthreads();
+      // fluxes from v and B0 at each staged cell, in place of B0 (the same
+      // expressions as the register staging)
+      for (int r = tid; r < NR; r += NT) {
+        const V vx = L[0][r], vy = L[1][r], vz = L[9][r];
+        const V b0x = L[3][r], b0y = L[4][r], b0z = L[5][r];
+        L[3][r] = -vy * b0x + vx * b0y;  // F3 = -vy box + vx boy
+        L[4][r] = -vx * b0z + vz * b0x;  // F2
+        L[5][r] = -vz * b0y + vy * b0z;  // F1
+      }
     } else if constexpr (MPX) {
       pre_load(k);
       mpoint_load(k);
@@ -497,7 +591,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       const uint32_t g = ij + (uint32_t)k * nxy;
       // xy derivatives and centre values from LDS
       V dvxdx = dx_lds(0), dvydy = dy_lds(1), gpx = dx_lds(2), gpy = dy_lds(2);
-      const int cc = (ty + 5) * LX + tx + HX;
+      const int cc = lidx(ty + 5, tx + HX);
       const V vx = L[0][cc], vy = L[1][cc];
       const V rho = PPF ? pc[0] : (MPX ? pm[0] : UF(RHO, g));
       const V vz = ZP ? (MPX ? pm[1] : UF(VZ, g)) : q[0][5], p = ZP ? L[2][cc] : q[1][5];
@@ -969,6 +1063,21 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP, VW>())) void
   stage_item<T, MHD, BG3D, TX, TY, ZP, VW>(Ap, logical);
 }
 
+// The FP64 MHD xy kernel with direct-to-LDS staging (16x16 tiles, even nx):
+// the staged planes take no registers and a tenth of the load instructions
+// (one 16-byte load per lane moves two cells); 2 waves per SIMD (its point
+// values and derivatives need ~230 registers; held to 3 waves it spilled
+// and ran 16% slower).  256^3 MHD xy kernel 1.787 -> 1.702 ms.
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_stage_dma(const StageArgs<T> A) {
+  const int nblk = gridDim.x, b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3, base = nblk >> 3, rem = nblk & 7;
+  const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
+  const CArgs<T> Ap = (CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr();
+  if (halted(Ap->halt)) return;
+  stage_item<T, true, true, 16, 16, true, 1, true>(Ap, logical);
+}
+
 // Many runs in one persistent launch: the work items of all runs form one
 // list (item_start[r] = first item of run r, item_start[nrun] = total),
 // handed out per XCD range by next_item.  The grid size must be a multiple
@@ -1120,6 +1229,13 @@ template <typename T>
 hipError_t launch_stage(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, const StageArgs<T>& A,
                         hipStream_t s) {
   if (!lanes_ok<T>(mhd, zp, vw)) return hipErrorInvalidValue;
+  if constexpr (sizeof(T) == 8) {
+    if (A.dma) {
+      if (!(mhd && zp && vw == 1 && tx == 16 && ty == 16 && A.nx % 2 == 0)) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_stage_dma<T>), dim3(A.nblk_xy * A.nchunk), dim3(256), 0, s, A);
+      return hipGetLastError();
+    }
+  }
   if constexpr (sizeof(T) == 4) {
     if (vw == 2) {
       if (mhd) return launch_tile<T, true, true, true, 2>(tx, ty, A, s);
