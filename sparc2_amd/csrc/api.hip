@@ -49,8 +49,8 @@ hipError_t launch_zero(void* p, long n, bool f32, hipStream_t st);
 template <typename T>
 hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty,
                               const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
-                              int* work, hipStream_t s);
-int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, int ncu);
+                              int* work, hipStream_t s, bool dma);
+int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, int ncu, bool dma);
 hipError_t launch_diag_reduce_batch(const void* jobs, int njobs, hipStream_t st);
 hipError_t launch_tick(int* c, int reset, hipStream_t st);
 hipError_t launch_copy(const void* in, void* out, long n, bool f32, hipStream_t st, const int* halt);
@@ -124,8 +124,10 @@ struct sparc_ctx {
   // MHD xy kernel; SPARC_LANE_POINTS=1 keeps one), else 1.  Tile widths
   // count lanes, so a tile spans vw*tx points.
   int vw = 1;
-  // FP64 MHD xy kernel: stage with direct-to-LDS loads where the tile is
-  // 16x16 and nx even (SPARC_MHD_DMA=0 keeps the register staging)
+  // MHD xy kernel: stage with direct-to-LDS loads where the tile is 16x16
+  // and each row has an even number of 8-byte lanes (nx even in FP64, a
+  // multiple of 4 with FP32's two-point lanes); SPARC_MHD_DMA=0 keeps the
+  // register staging
   bool dma = false;
   void* dzbuf = nullptr;
   void* bg[B_COUNT] = {};
@@ -424,7 +426,7 @@ int sparc_hip_create(const sparc_cfg* cfg, sparc_ctx** out) {
     const char* lp = getenv("SPARC_LANE_POINTS");
     c->vw = (c->f32 && q.nx % 2 == 0 && (c->zp || !c->mhd) && !(lp && lp[0] == '1')) ? 2 : 1;
     const char* dm = getenv("SPARC_MHD_DMA");
-    c->dma = !c->f32 && c->zp && q.nx % 2 == 0 && !(dm && dm[0] == '0');
+    c->dma = c->zp && (c->f32 ? (c->vw == 2 && q.nx % 4 == 0) : q.nx % 2 == 0) && !(dm && dm[0] == '0');
   }
   c->nv = c->mhd ? 8 : 5;
   c->npsi = c->mhd ? 6 : 2;
@@ -1184,7 +1186,7 @@ int sparc_hip_tile(sparc_ctx* c, int tile[3], long* nblocks) {
 // ===========================================================================
 namespace {
 struct BatchGroup {
-  bool mhd = false, bg3d = false, zp = false;
+  bool mhd = false, bg3d = false, zp = false, dma = false;
   int vw = 1, tx = 0, ty = 0, nblk = 0, znblk = 0;
   std::vector<int> members;
   std::vector<int> zkc;        // z-pass chunk depth per member in this batch
@@ -1261,7 +1263,7 @@ static int batch_enqueue_step(sparc_batch* b) {
       if (g.zp) e = launch_zpass_batch<T>(args, g.zitem_start, nm, g.znblk, b->work, b->st);
       if (e == hipSuccess)
         e = launch_stage_batch<T>(g.mhd, g.bg3d, g.zp, g.vw, g.tx, g.ty, args, g.item_start, nm, g.nblk,
-                                  b->work, b->st);
+                                  b->work, b->st, g.dma);
       if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch stage launch: ") + hipGetErrorString(e));
     }
   if (launch_diag_reduce_batch(b->jobs, (int)b->runs.size(), b->st) != hipSuccess ||
@@ -1297,6 +1299,11 @@ static int batch_filter(sparc_batch* b, sparc_ctx* c, int time) {
   if (e != hipSuccess) return fail(SPARC_E_HIP, std::string("batch filter: ") + hipGetErrorString(e));
   return SPARC_OK;
 }
+
+// The batched MHD xy kernel stages with direct-to-LDS loads in FP64 only:
+// the FP32 persistent kernel then needs 256 registers and spills (the
+// 64-run population: 2.15 -> 2.10 GPt/s; FP64 batch 1.24 -> 1.29 GPt/s).
+static bool batch_dma(const sparc_ctx* c) { return c->dma && !c->f32 && c->tx == 16 && c->ty == 16; }
 
 int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, sparc_batch** out) {
   if (!runs || nrun < 1 || !out) return fail(SPARC_E_INVALID, "batch needs at least one run");
@@ -1336,7 +1343,7 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
     BatchGroup* g = nullptr;
     for (auto& h : b->groups)
       if (h.mhd == c->mhd && h.bg3d == c->bg3d && h.zp == c->zp && h.vw == c->vw && h.tx == c->tx &&
-          h.ty == c->ty)
+          h.ty == c->ty && h.dma == batch_dma(c))
         g = &h;
     if (!g) {
       b->groups.emplace_back();
@@ -1347,6 +1354,7 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
       g->vw = c->vw;
       g->tx = c->tx;
       g->ty = c->ty;
+      g->dma = batch_dma(c);
     }
     g->members.push_back(r);
   }
@@ -1363,7 +1371,7 @@ int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, spar
     if ((rc = dalloc((void**)&g.item_start, sizeof(int) * start.size()))) return bail(rc);
     if (hipMemcpy(g.item_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(SPARC_E_HIP, "item table upload failed"));
-    g.nblk = std::min(stage_batch_blocks(b->f32, g.mhd, g.bg3d, g.zp, g.vw, g.tx, g.ty, ncu),
+    g.nblk = std::min(stage_batch_blocks(b->f32, g.mhd, g.bg3d, g.zp, g.vw, g.tx, g.ty, ncu, g.dma),
                       (start.back() + 7) / 8 * 8);
     if (g.zp) {
       // z-pass chunks sized for the whole group rather than per run: about
@@ -1520,7 +1528,7 @@ static int batch_time_stage_t(sparc_batch* b, int reps, double* ms) {
       if (g.zp) e = launch_zpass_batch<T>(args, g.zitem_start, nm, g.znblk, b->work, b->st);
       if (e == hipSuccess)
         e = launch_stage_batch<T>(g.mhd, g.bg3d, g.zp, g.vw, g.tx, g.ty, args, g.item_start, nm, g.nblk,
-                                  b->work, b->st);
+                                  b->work, b->st, g.dma);
     }
   }
   if (e == hipSuccess) e = hipEventRecord(e1, b->st);

@@ -45,14 +45,15 @@ constexpr int stage_waves() {
 template <typename T, bool MHD, bool BG3D, int TX, int TY, bool ZP, int VW, bool DMA = false>
 __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   static_assert(MHD || !ZP, "the z pass serves the MHD stage only");
-  static_assert(!DMA || (MHD && ZP && VW == 1 && sizeof(T) == 8 && TX == 16 && TY == 16),
-                "direct-to-LDS staging: the FP64 MHD xy kernel on 16x16 tiles");
+  static_assert(!DMA || (MHD && ZP && TX == 16 && TY == 16 && VW * sizeof(T) == 8),
+                "direct-to-LDS staging: the MHD xy kernel on 16x16 tiles of 8-byte lanes");
   static_assert(VW == 1 || (sizeof(T) == 4 && (ZP || !MHD)), "two-point lanes: FP32 quiet or MHD xy");
   using V = typename Lane<T, VW>::V;
   constexpr int NT = TX * TY;
-  // x halo in lane columns: 5 points, rounded up to whole lanes (DMA: 6, so
-  // the 16-byte cell pairs of a row never straddle a periodic wrap)
-  constexpr int HX = DMA ? 6 : (VW == 1 ? 5 : 3);
+  // x halo in lane columns: 5 points, rounded up to whole lanes (DMA: to an
+  // even number of lanes, so the 16-byte pairs of lane cells of a row never
+  // straddle a periodic wrap)
+  constexpr int HX = DMA ? (VW == 1 ? 6 : 4) : (VW == 1 ? 5 : 3);
   constexpr int LX = TX + 2 * HX, LY = TY + 10;
   constexpr int NR = TY * LX + 10 * TX;  // cross-shaped staging region
   constexpr int NS = (NR + NT - 1) / NT;
@@ -132,14 +133,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     s_lds[s] = r < NR ? row * LX + col : LY * LX;
   }
 
-  // DMA plan: one wave load moves 64 x 16 bytes -- two adjacent x cells per
-  // lane -- into 128 consecutive packed LDS cells; chunk c of a plane is
-  // packed cells c*128 .. c*128+127.  Per lane and chunk: the element offset
-  // of its cell pair in the plane, or ~0 (the load reads 0: outside the
-  // domain of a non-periodic axis, or past the region).  With nx even, HX
-  // even and the tile's first x a multiple of 16, a pair starts at an even
-  // x, so it lies wholly inside or outside the domain and never straddles
-  // a periodic wrap.
+  // DMA plan: one wave load moves 64 x 16 bytes -- two adjacent lane cells
+  // (8 bytes each: one FP64 point, or two FP32 points) per lane -- into 128
+  // consecutive packed LDS cells; chunk c of a plane is packed cells c*128 ..
+  // c*128+127.  Per lane and chunk: the element offset of its cell pair in
+  // the plane, or ~0 (the load reads 0: outside the domain of a
+  // non-periodic axis, or past the region).  With an even number of lane
+  // columns per row (nx even in FP64, a multiple of 4 in FP32), HX even and
+  // the tile's first lane column a multiple of 16, a pair starts at an even
+  // lane column, so it lies wholly inside or outside the domain and never
+  // straddles a periodic wrap.
   constexpr int NCH = DMA ? LP / 128 : 1;
   uint32_t d_off[NCH];
   if constexpr (DMA) {
@@ -157,15 +160,16 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         row = hr < 5 ? hr : hr + TY;
         col = HX + r2 % TX;
       }
-      int gi = i0 - HX + col, gj = j0 - 5 + row;
+      const int nxl = nx / VW;
+      int gi = i0 / VW - HX + col, gj = j0 - 5 + row;
       bool ok = r < NR;
       if (periodic) {
-        gi = wrap(gi, nx);
+        gi = wrap(gi, nxl);
         gj = wrap(gj, ny);
       } else {
-        ok = ok && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+        ok = ok && gi >= 0 && gi < nxl && gj >= 0 && gj < ny;
       }
-      d_off[c] = ok ? (uint32_t)(gi + nx * gj) : 0xFFFFFFFFu;
+      d_off[c] = ok ? (uint32_t)(VW * gi + nx * gj) : 0xFFFFFFFFu;
     }
   }
 
@@ -1063,11 +1067,17 @@ __global__ __launch_bounds__(TX* TY, (stage_waves<T, MHD, BG3D, ZP, VW>())) void
   stage_item<T, MHD, BG3D, TX, TY, ZP, VW>(Ap, logical);
 }
 
-// The FP64 MHD xy kernel with direct-to-LDS staging (16x16 tiles, even nx):
-// the staged planes take no registers and a tenth of the load instructions
-// (one 16-byte load per lane moves two cells); 2 waves per SIMD (its point
-// values and derivatives need ~230 registers; held to 3 waves it spilled
-// and ran 16% slower).  256^3 MHD xy kernel 1.787 -> 1.702 ms.
+// The MHD xy kernel with direct-to-LDS staging (16x16 tiles of 8-byte
+// lanes: FP64, or FP32 with two points per lane; nx even, in FP32 a
+// multiple of 4): the staged planes take no registers and a tenth of the
+// load instructions (one 16-byte load per lane moves two cells); 2 waves per
+// SIMD (the point values and derivatives need ~230 registers; held to 3
+// waves the FP64 kernel spilled and ran 16% slower).  256^3 FP64 MHD xy
+// kernel 1.787 -> 1.702 ms.
+template <typename T>
+constexpr int dma_vw() {
+  return sizeof(T) == 4 ? 2 : 1;
+}
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_stage_dma(const StageArgs<T> A) {
   const int nblk = gridDim.x, b = blockIdx.x;
@@ -1075,7 +1085,25 @@ __global__ __launch_bounds__(256, 2) void k_stage_dma(const StageArgs<T> A) {
   const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
   const CArgs<T> Ap = (CArgs<T>)__builtin_amdgcn_kernarg_segment_ptr();
   if (halted(Ap->halt)) return;
-  stage_item<T, true, true, 16, 16, true, 1, true>(Ap, logical);
+  stage_item<T, true, true, 16, 16, true, dma_vw<T>(), true>(Ap, logical);
+}
+
+// The same, persistent over the work items of a batch group.
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_stage_batch_dma(const StageArgs<T>* __restrict__ runs,
+                                                             const int* __restrict__ item_start,
+                                                             int nrun, int* __restrict__ work) {
+  __shared__ int slot;
+  const int total = item_start[nrun];
+  const int xcd = blockIdx.x & 7;
+  const int lo = (int)((long)total * xcd / 8), hi = (int)((long)total * (xcd + 1) / 8);
+  int r = 0;
+  for (int item = next_item(work, lo, hi, &slot); item >= 0; item = next_item(work, lo, hi, &slot)) {
+    while (r + 1 < nrun && item_start[r + 1] <= item) ++r;
+    const CArgs<T> Ap = (CArgs<T>)(runs + r);
+    if (!halted(Ap->halt)) stage_item<T, true, true, 16, 16, true, dma_vw<T>(), true>(Ap, item - item_start[r]);
+  }
+  leave(work);
 }
 
 // Many runs in one persistent launch: the work items of all runs form one
@@ -1157,8 +1185,13 @@ static bool lanes_ok(bool mhd, bool zp, int vw) {
 template <typename T>
 hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty,
                               const StageArgs<T>* runs, const int* item_start, int nrun, int nblk,
-                              int* work, hipStream_t s) {
+                              int* work, hipStream_t s, bool dma) {
   if (!lanes_ok<T>(mhd, zp, vw) || nblk % 8 != 0) return hipErrorInvalidValue;
+  if (dma) {
+    if (!(mhd && zp && vw == dma_vw<T>() && tx == 16 && ty == 16)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_stage_batch_dma<T>), dim3(nblk), dim3(256), 0, s, runs, item_start, nrun, work);
+    return hipGetLastError();
+  }
 #define SPARC_BT(M, B, Z, V) launch_batch_tile<T, M, B, Z, V>(tx, ty, runs, item_start, nrun, nblk, work, s)
   if constexpr (sizeof(T) == 4) {
     if (vw == 2) {
@@ -1175,9 +1208,9 @@ hipError_t launch_stage_batch(bool mhd, bool bg3d, bool zp, int vw, int tx, int 
 }
 
 template hipError_t launch_stage_batch<double>(bool, bool, bool, int, int, int, const StageArgs<double>*,
-                                               const int*, int, int, int*, hipStream_t);
+                                               const int*, int, int, int*, hipStream_t, bool);
 template hipError_t launch_stage_batch<float>(bool, bool, bool, int, int, int, const StageArgs<float>*,
-                                              const int*, int, int, int*, hipStream_t);
+                                              const int*, int, int, int*, hipStream_t, bool);
 
 // Persistent grid for a batch launch: the blocks of this kernel instance that
 // are resident at once (the runtime's occupancy for its registers and LDS),
@@ -1220,7 +1253,9 @@ static int batch_blocks_t(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, 
   return batch_blocks_tile<T, false, false, false, 1>(tx, ty, ncu);
 }
 
-int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, int ncu) {
+int stage_batch_blocks(bool f32, bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, int ncu, bool dma) {
+  if (dma)
+    return f32 ? resident_grid(k_stage_batch_dma<float>, 256, ncu) : resident_grid(k_stage_batch_dma<double>, 256, ncu);
   return f32 ? batch_blocks_t<float>(mhd, bg3d, zp, vw, tx, ty, ncu)
              : batch_blocks_t<double>(mhd, bg3d, zp, vw, tx, ty, ncu);
 }
@@ -1229,12 +1264,11 @@ template <typename T>
 hipError_t launch_stage(bool mhd, bool bg3d, bool zp, int vw, int tx, int ty, const StageArgs<T>& A,
                         hipStream_t s) {
   if (!lanes_ok<T>(mhd, zp, vw)) return hipErrorInvalidValue;
-  if constexpr (sizeof(T) == 8) {
-    if (A.dma) {
-      if (!(mhd && zp && vw == 1 && tx == 16 && ty == 16 && A.nx % 2 == 0)) return hipErrorInvalidValue;
-      hipLaunchKernelGGL((k_stage_dma<T>), dim3(A.nblk_xy * A.nchunk), dim3(256), 0, s, A);
-      return hipGetLastError();
-    }
+  if (A.dma) {
+    if (!(mhd && zp && vw == dma_vw<T>() && tx == 16 && ty == 16 && A.nx % (2 * vw) == 0))
+      return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_stage_dma<T>), dim3(A.nblk_xy * A.nchunk), dim3(256), 0, s, A);
+    return hipGetLastError();
   }
   if constexpr (sizeof(T) == 4) {
     if (vw == 2) {

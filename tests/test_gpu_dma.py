@@ -1,12 +1,14 @@
-"""The FP64 MHD xy kernel's two staging paths give the same bits.
+"""The MHD xy kernel's two staging paths give the same bits.
 
-On 16x16 tiles with even nx the kernel stages its planes with direct-to-LDS
-buffer loads (two cells per lane, packed cross-shaped LDS region, fluxes
-formed in LDS); SPARC_MHD_DMA=0 (read when a context is made) keeps the
-register staging.  Both read the same values and do the same arithmetic, so
-states and CPML bases after a driver loop with filters must be identical,
-on a 6-face CPML grid, a periodic one, and (odd nx, where the register path
-is used either way) a grid the direct path refuses.
+On 16x16 tiles whose rows hold an even number of 8-byte lanes (nx even in
+FP64, a multiple of 4 in FP32 with two points per lane) the kernel stages
+its planes with direct-to-LDS buffer loads (two lane cells per lane, packed
+cross-shaped LDS region, fluxes formed in LDS); SPARC_MHD_DMA=0 (read when a
+context is made) keeps the register staging.  Both read the same values and
+do the same arithmetic, so states and CPML bases after a driver loop with
+filters must be identical, on a 6-face CPML grid, a periodic one, and grids
+the direct path refuses (odd nx; in FP32 nx = 2 mod 4), where the register
+path runs either way.
 """
 import os
 
@@ -21,6 +23,7 @@ CASES = {
     "six_faces": lambda: mhd_cfg(nx=48, ny=38, nz=40),
     "periodic": lambda: mhd_cfg(nx=64, ny=32, nz=36, periodic=True),
     "odd_nx": lambda: mhd_cfg(nx=47, ny=38, nz=36),
+    "nx_2_mod_4": lambda: mhd_cfg(nx=54, ny=38, nz=36),
 }
 
 
@@ -42,15 +45,17 @@ def _run(cfg, bg, a, psi, dma, nsteps=9):
     return out
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_direct_to_lds_staging_equals_register_staging(case):
-    cfg = CASES[case]()
+def test_direct_to_lds_staging_equals_register_staging(case, precision):
+    cfg = CASES[case]().replace(precision=precision)
     bg = background(cfg)
     a, psi = random_state(cfg, seed=29)
     assert cfg.xyfreq <= 9 and cfg.zfreq <= 9     # filters inside the loop
     got, gpsi = _run(cfg, bg, a, psi, dma=True)
     ref, rpsi = _run(cfg, bg, a, psi, dma=False)
-    print(f"{case} ({cfg.nx}x{cfg.ny}x{cfg.nz}): max |diff| {np.abs(got - ref).max():.3e}", flush=True)
+    print(f"{case} {precision} ({cfg.nx}x{cfg.ny}x{cfg.nz}): max |diff| {np.abs(got - ref).max():.3e}",
+          flush=True)
     np.testing.assert_array_equal(got, ref)
     for f in rpsi:
         np.testing.assert_array_equal(gpsi[f], rpsi[f])
