@@ -802,7 +802,7 @@ static void set_stage(sparc_ctx* c, StageArgs<T>& A, int s) {
   A.beta_lo = A.last ? (T)0 : (T)(c->cfg.betas[s] - (double)A.beta_next);
   for (int f = 0; f < 8; ++f) {
     A.u[f] = (const T*)c->pool[uin] + (size_t)f * c->fs;
-    A.un[f] = A.last ? nullptr : (T*)c->pool[uout] + (size_t)f * c->fs;
+    A.un[f] = A.last ? A.a[f] : (T*)c->pool[uout] + (size_t)f * c->fs;
   }
 }
 

@@ -162,7 +162,7 @@ template <typename T> struct StageArgs {
   const int* tstep;
   const T* u[8];
   T* a[8];
-  T* un[8];
+  T* un[8];           // what the stage writes: the next stage value, or a itself on the last stage
   uint16_t* comp[8];  // Kahan compensation (fp32 only, bfloat16), updated in place on the last stage
   T* dz[6];    // MHD z-pass output: z derivatives in CPML family order
   const T* damp[3];  // FFT damping terms of vx, vy, vz (null: no damping)

@@ -502,8 +502,26 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   // background slot at the point in pm (MHD xy kernel)
   auto PMB = [&](int q2) -> V { return pm[MP_BG + q2]; };
 
+  // Source factors constant along z, loaded once per work item: the pulse's
+  // amp * sx(i) * sy(j), or forcing(x, y) with PULSE = 0; each plane then
+  // multiplies by its z factor, a scalar load.  (Loaded per plane, their
+  // wait would also wait for the next plane's prefetched loads, which
+  // complete in order before them.)
+  const int pdir = Ap->pulse_dir;
+  V srcxy = 0;
+  if (Ap->fmap) {
+    srcxy = ldv<V>(Buf<T>(Ap->fmap, nxy), (uint32_t)(i + nx * j) * W);
+  } else if (pdir >= 1 && pdir <= 3) {
+    const T amp = Ap->amp_tab ? Ap->amp_tab[*Ap->tstep] : Ap->src_amp;
+    srcxy = amp * ldv<V>(Buf<T>(Ap->sx, nx), (uint32_t)i * W) * Buf<T>(Ap->sy, ny)[j];
+  }
+
   if constexpr (SPF) pre_load(k0);
   if constexpr (PPF) point_load(k0, pn);
+  // the first plane's loads complete before the loop, so that on entry, as
+  // on every later pass, the loop-top wait has only stores left to wait on
+  // (vmcnt 0, expcnt and lgkmcnt not waited on)
+  if constexpr (SPF) __builtin_amdgcn_s_waitcnt(0x0F70);
   for (int k = k0; k < k1; ++k) {
     SPARC_REFRESH(Ap);
     L = LB[NB == 2 ? (k & 1) : 0];
@@ -514,8 +532,11 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       // part of it to the first use; the in-order load counter then makes
       // that later wait cover the next plane's loads too, and the prefetch
       // stops overlapping the compute (quiet 512x512x300 FP64 stage 2.21 ->
-      // 2.63 ms).  0x0F70: vmcnt 0, expcnt and lgkmcnt not waited on.
-      __builtin_amdgcn_s_waitcnt(0x0F70);
+      // 2.63 ms).  The previous plane's NV output stores, the youngest
+      // operations, stay in flight.  0x0F70 | NV: vmcnt NV, expcnt and
+      // lgkmcnt not waited on.
+      static_assert(NV < 16, "vmcnt field");
+      __builtin_amdgcn_s_waitcnt(0x0F70 | NV);
       pre_store();
       if constexpr (PPF) {
 #pragma unroll
@@ -591,8 +612,17 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     __syncthreads();
     SPARC_REFRESH(Ap);
 
+    // The plane's outputs (next stage value, or the new base and its FP32
+    // compensation) are stored after the point block by every wave, a lane
+    // outside the grid at an offset past the buffer's range (dropped): each
+    // iteration then ends with the same stores on every path, and the next
+    // plane's wait for its prefetched loads can leave them in flight (loads
+    // and stores complete in issue order).
+    const uint32_t g = ij + (uint32_t)k * nxy;
+    V outv[NV], outc[NV];
+#pragma unroll
+    for (int f = 0; f < NV; ++f) outv[f] = outc[f] = 0;
     if (active) {
-      const uint32_t g = ij + (uint32_t)k * nxy;
       // xy derivatives and centre values from LDS
       V dvxdx = dx_lds(0), dvydy = dy_lds(1), gpx = dx_lds(2), gpy = dy_lds(2);
       const int cc = lidx(ty + 5, tx + HX);
@@ -771,13 +801,10 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       }
       // pulse source (driver.f90:137-148 separable form; RHS.f90:138-145,632-639),
       // or with PULSE = 0 forcing(x, y) * source_dep(z) on vz (RHS.f90:146-147, 640-641)
-      const int pdir = Ap->pulse_dir;
       if (Ap->fmap) {
-        K[VZ] += ldv<V>(Buf<T>(Ap->fmap, nxy), (uint32_t)(i + nx * j) * W) * ((CPtr<T>)Ap->sdep)[k];
+        K[VZ] += srcxy * ((CPtr<T>)Ap->sdep)[k];
       } else if (pdir >= 1 && pdir <= 3) {
-        const T amp = Ap->amp_tab ? Ap->amp_tab[*Ap->tstep] : Ap->src_amp;
-        const V s = amp * ldv<V>(Buf<T>(Ap->sx, nx), (uint32_t)i * W) * Buf<T>(Ap->sy, ny)[j] *
-                    ((CPtr<T>)Ap->sz)[k];
+        const V s = srcxy * ((CPtr<T>)Ap->sz)[k];
         if (pdir == 1) K[VX] += s;
         else if (pdir == 2) K[VY] += s;
         else K[VZ] += s;
@@ -801,31 +828,27 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         // next stage value, step.f90:33
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
-          const Buf<T> uf(Ap->un[f], fs);
           const V a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : ldv<V>(Buf<T>(Ap->a[f], fs), g * W));
-          stv<CPW>(uf, g, a0 + scl(bnext, Ap->beta_lo, K[f]));
+          outv[f] = a0 + scl(bnext, Ap->beta_lo, K[f]);
         }
       } else {
-        V an[NV];
+        V (&an)[NV] = outv;
 #pragma unroll
         for (int f = 0; f < NV; ++f) {
-          const Buf<T> af(Ap->a[f], fs);
-          const V a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : ldv<V>(af, g * W));
+          const V a0 = PPF ? pc[1 + f] : (MPX ? pm[MP_A + f] : ldv<V>(Buf<T>(Ap->a[f], fs), g * W));
           if constexpr (sizeof(T) == 4) {
             // compensated update: a - c carries the bits a + dt K loses
-            const Buf<uint16_t> cf(Ap->comp[f], fs);
             V c0;
             if constexpr (NPC > 0) c0 = pc[IC + f];
             else if constexpr (MPX) c0 = pm[MP_C + f];
-            else c0 = ldcomp<V>(cf, g * W);
+            else c0 = ldcomp<V>(Buf<uint16_t>(Ap->comp[f], fs), g * W);
             const V y = scl(dt, Ap->dt_lo, K[f]) - c0;
             const V t = a0 + y;
-            stcomp<CPW>(cf, g, (t - a0) - y);
+            outc[f] = (t - a0) - y;
             an[f] = t;
           } else {
             an[f] = a0 + dt * K[f];
           }
-          stv<CPW>(af, g, an[f]);
         }
         // diagnostics: KE norm on the new state, minima on the stage-5 state
         V p0v;
@@ -842,6 +865,20 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           mp = fmin(mp, (double)lane_pt(p, c) + (double)lane_pt(p0v, c));
         }
       }
+    }
+    {
+      // element offsets whose byte offsets are Buf::kOut (past every range)
+      const uint32_t ge = active ? g : Buf<T>::kOut / (uint32_t)sizeof(T);
+      const uint32_t gc = active ? g : Buf<T>::kOut / 2u;
+      // un is a itself on the last stage
+      if constexpr (sizeof(T) == 4) {
+        if (last) {
+#pragma unroll
+          for (int f = 0; f < NV; ++f) stcomp<CPW>(Buf<uint16_t>(Ap->comp[f], fs), gc, outc[f]);
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < NV; ++f) stv<CPW>(Buf<T>(Ap->un[f], fs), ge, outv[f]);
     }
     // single buffer: the next plane's staging must wait for these reads
     if constexpr (NB == 1) __syncthreads();
