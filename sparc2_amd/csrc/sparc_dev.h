@@ -343,6 +343,56 @@ struct RowSumF2 {
 template <typename V> struct RowSumOf { using type = RowSum<V>; };
 template <> struct RowSumOf<F2> { using type = RowSumF2; };
 
+// The row tables copied into LDS, for the stage kernels: there the row of a
+// one-sided closure differs between the lanes of a wave, so reading the
+// constant tables takes vector memory loads, and their wait also covers
+// every load in flight (the point values of the plane); LDS reads do not.
+// FP32: hi and lo of the coefficient pairs.
+template <typename T> struct RowTab {
+  T c[5 * 11];
+  T l[sizeof(T) == 4 ? 5 * 11 : 1];
+  // every thread of the block calls this; a barrier must follow
+  __device__ __forceinline__ void fill(int tid, int nt) {
+    for (int q = tid; q < 5 * 11; q += nt) {
+      if constexpr (sizeof(T) == 8) {
+        c[q] = DPB_D[q / 11][q % 11];
+      } else {
+        c[q] = DPB_F.hi[q / 11][q % 11];
+        l[q] = DPB_F.lo[q / 11][q % 11];
+      }
+    }
+  }
+};
+// RowSum over an LDS copy of the tables (the same products and sums)
+template <typename T> struct RowSumT {
+  const RowTab<T>& t;
+  T p = 0, s = 0;
+  __device__ __forceinline__ explicit RowSumT(const RowTab<T>& tab) : t(tab) {}
+  __device__ __forceinline__ void add(T v, int r, int m) {
+    if constexpr (sizeof(T) == 8) {
+      p += v * t.c[r * 11 + m];
+    } else {
+      p += v * t.c[r * 11 + m];
+      s += v * t.l[r * 11 + m];
+    }
+  }
+  __device__ __forceinline__ T sum() const {
+    if constexpr (sizeof(T) == 8) return p;
+    else return p + s;
+  }
+};
+struct RowSumTF2 {
+  RowSumT<float> a, b;
+  __device__ __forceinline__ explicit RowSumTF2(const RowTab<float>& tab) : a(tab), b(tab) {}
+  __device__ __forceinline__ void add(F2 v, int r, int m) {
+    a.add(v.x, r, m);
+    b.add(v.y, r, m);
+  }
+  __device__ __forceinline__ F2 sum() const { return F2(a.sum(), b.sum()); }
+};
+template <typename V> struct RowSumTOf { using type = RowSumT<V>; };
+template <> struct RowSumTOf<F2> { using type = RowSumTF2; };
+
 // point c of a lane value
 template <typename T> __device__ __forceinline__ T lane_pt(T v, int) { return v; }
 __device__ __forceinline__ float lane_pt(F2 v, int c) { return c ? v.y : v.x; }
@@ -430,12 +480,18 @@ __device__ __forceinline__ void pml_face(FR& F, uint32_t fp, V (&v)[NF], const i
   const uint32_t bo = fp * (uint32_t)sizeof(T);
   const V ca = ldv<V, CPL>(Buf<T>(F.a, n), bo), cb = ldv<V, CPL>(Buf<T>(F.b, n), bo),
           ck = ldv<V, CPL>(Buf<T>(F.kinv, n), bo);
+  // the stage value is the base itself on the first stage: its load then
+  // reads nothing (a branch around it would split the loads into one round
+  // trip per family)
+  const uint32_t bs = first ? Buf<T>::kOut : bo;
   V base[NF], ps[NF];
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
     base[i] = ldv<V, CPL>(Buf<T>(F.psi + (size_t)fam[i] * n, n), bo);
-    ps[i] = first ? base[i] : ldv<V, CPL>(Buf<T>(F.pst + (size_t)fam[i] * n, n), bo);
+    ps[i] = ldv<V, CPL>(Buf<T>(F.pst + (size_t)fam[i] * n, n), bs);
   }
+#pragma unroll
+  for (int i = 0; i < NF; ++i) ps[i] = first ? base[i] : ps[i];
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
     const V kn = ca * v[i] + cb * ps[i];

@@ -78,8 +78,11 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
   constexpr int LP = DMA ? (NR + 127) / 128 * 128 : LY * LX + 1;
   __shared__ V LB[NB][NL][LP];
   __shared__ double rs[3][NT / 64 + 1];
+  __shared__ RowTab<T> rtab;
 
   const int tid = threadIdx.x;
+  rtab.fill(tid, NT);
+  __syncthreads();
   const int tx = tid % TX, ty = tid / TX;
   const int nx = Ap->nx, ny = Ap->ny, nz = Ap->nz;
   const uint32_t fs = (uint32_t)Ap->fs;
@@ -340,7 +343,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
 #pragma unroll
       for (int c = 0; c < VW; ++c) {
         const int p = i + c;
-        RowSum<T> acc;
+        RowSumT<T> acc(rtab);
         if (p < 5) {
 #pragma unroll
           for (int m = 0; m < 11; ++m) acc.add(rp[min(m - i0 + XO, RW - 1)], p, m);
@@ -359,14 +362,14 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
     const V* pl = L[f];
     const int cx = tx + HX;
     if (!periodic && j < 5) {
-      typename RowSumOf<V>::type e;
+      typename RowSumTOf<V>::type e(rtab);
 #pragma unroll
       for (int m = 0; m < 11; ++m) e.add(pl[lidx(min(max(m - j0 + 5, 0), LY - 1), cx)], j, m);
       return scl(sys, syl, e.sum());
     }
     if (!periodic && j >= ny - 5) {
       const int r = ny - 1 - j;
-      typename RowSumOf<V>::type e;
+      typename RowSumTOf<V>::type e(rtab);
 #pragma unroll
       for (int m = 0; m < 11; ++m) e.add(pl[lidx(min(max(ny - 1 - m - j0 + 5, 0), LY - 1), cx)], r, m);
       return -scl(sys, syl, e.sum());
@@ -601,7 +604,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           const int r = nz - 1 - k;
 #pragma unroll
           for (int f = 0; f < NQ; ++f) {
-            typename RowSumOf<V>::type acc;
+            typename RowSumTOf<V>::type acc(rtab);
 #pragma unroll
             for (int m = 0; m < 11; ++m) acc.add(q[f][m], r, (r + 16 - m) % 11);
             dz[f] = -scl(st, stl, acc.sum());
