@@ -452,6 +452,27 @@ __device__ __forceinline__ V ldcomp(const Buf<uint16_t>& b, uint32_t fbo) {
     return bf2f(__builtin_amdgcn_raw_buffer_load_b16(b.r, bo, 0, CP));
   }
 }
+// The same load with the bits kept raw (in .x for F2, zero-extended for
+// float), widened by comp_of_raw where the value is used: widening at the
+// load would wait for it there, and a prefetch must not be waited on early.
+template <typename V, int CP = 0>
+__device__ __forceinline__ V ldcomp_raw(const Buf<uint16_t>& b, uint32_t fbo) {
+  const int bo = (int)(fbo == Buf<float>::kOut ? fbo : fbo >> 1);
+  if constexpr (sizeof(V) == 8) {
+    return V(__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, bo, 0, CP)), 0.0f);
+  } else {
+    static_assert(sizeof(V) == 4, "a compensation lane of one or two FP32 points");
+    return __builtin_bit_cast(float, (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(b.r, bo, 0, CP));
+  }
+}
+template <typename V> __device__ __forceinline__ V comp_of_raw(V r) {
+  if constexpr (sizeof(V) == 8) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, r.x);
+    return V(bf2f((uint16_t)(u & 0xFFFFu)), bf2f((uint16_t)(u >> 16)));
+  } else {
+    return bf2f((uint16_t)__builtin_bit_cast(uint32_t, r));
+  }
+}
 template <int CP = 0, typename V>
 __device__ __forceinline__ void stcomp(const Buf<uint16_t>& b, uint32_t e, V v) {
   if constexpr (sizeof(V) == 8) {

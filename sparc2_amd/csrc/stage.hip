@@ -461,7 +461,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
       if (last) {
         if constexpr (NPC > 0) {
 #pragma unroll
-          for (int f = 0; f < NV; ++f) dst[IC + f] = ldcomp<V, CPL>(Buf<uint16_t>(Ap->comp[f], fs), bo);
+          for (int f = 0; f < NV; ++f) dst[IC + f] = ldcomp_raw<V, CPL>(Buf<uint16_t>(Ap->comp[f], fs), bo);
         }
         if constexpr (NPP > 0) dst[IP] = LDS1(Ap->p0, bo);
       }
@@ -497,7 +497,7 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
         pm[MP_P0] = LDS1(Ap->p0, bo);
         if constexpr (sizeof(T) == 4) {
 #pragma unroll
-          for (int f = 0; f < 8; ++f) pm[MP_C + f] = ldcomp<V, CPL>(Buf<uint16_t>(Ap->comp[f], fs), bo);
+          for (int f = 0; f < 8; ++f) pm[MP_C + f] = ldcomp_raw<V, CPL>(Buf<uint16_t>(Ap->comp[f], fs), bo);
         }
       }
     }
@@ -842,8 +842,8 @@ __device__ __forceinline__ void stage_item(CArgs<T> Ap, const int logical) {
           if constexpr (sizeof(T) == 4) {
             // compensated update: a - c carries the bits a + dt K loses
             V c0;
-            if constexpr (NPC > 0) c0 = pc[IC + f];
-            else if constexpr (MPX) c0 = pm[MP_C + f];
+            if constexpr (NPC > 0) c0 = comp_of_raw(pc[IC + f]);
+            else if constexpr (MPX) c0 = comp_of_raw(pm[MP_C + f]);
             else c0 = ldcomp<V>(Buf<uint16_t>(Ap->comp[f], fs), g * W);
             const V y = scl(dt, Ap->dt_lo, K[f]) - c0;
             const V t = a0 + y;
