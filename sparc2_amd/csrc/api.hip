@@ -1302,7 +1302,7 @@ static int batch_filter(sparc_batch* b, sparc_ctx* c, int time) {
 
 // The batched MHD xy kernel stages with direct-to-LDS loads in FP64 only:
 // the FP32 persistent kernel then needs 256 registers and spills (the
-// 64-run population: 2.15 -> 2.10 GPt/s; FP64 batch 1.24 -> 1.29 GPt/s).
+// 64-run population: 2.75 -> 2.56 GPt/s; FP64 batch 1.24 -> 1.29 GPt/s).
 static bool batch_dma(const sparc_ctx* c) { return c->dma && !c->f32 && c->tx == 16 && c->ty == 16; }
 
 int sparc_hip_batch_create(sparc_ctx* const* runs, int nrun, int use_graph, sparc_batch** out) {
